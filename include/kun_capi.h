@@ -1,0 +1,151 @@
+/* C ABI of libkunhip.so, the MI355X device runtime of the factor pipeline.
+ *
+ * The first block has the names, argument meaning and lifetime rules of the
+ * reference C API (KunQuant cpp/Kun/CApi.h); the comment of each entry
+ * point cites the declaration it replaces. Buffers passed to these entry
+ * points are caller-owned host memory, exactly as in the reference: the
+ * runtime copies inputs to HBM, runs the stage kernels and copies outputs
+ * back before returning.
+ *
+ * The second block holds extensions that have no counterpart in
+ * CApi.h: module introspection (the reference reads its Module struct
+ * directly, PyBinding.cpp:182-220), error reporting (the reference throws
+ * C++ exceptions), and entry points taking device-resident buffers.
+ */
+#ifndef KUN_CAPI_H
+#define KUN_CAPI_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KUN_API __attribute__((visibility("default")))
+
+typedef void *KunExecutorHandle;
+typedef void *KunLibraryHandle;
+typedef void *KunModuleHandle;
+typedef void *KunBufferNameMapHandle;
+typedef void *KunStreamContextHandle;
+
+#define KUN_API_VERSION 1
+
+typedef enum {
+    KUN_INIT_NONE = 0,
+    KUN_INIT_FILE,
+    KUN_INIT_MEMORY,
+} KunStateBufferKind;
+
+typedef enum {
+    KUN_SUCCESS = 0,
+    KUN_INIT_ERROR,
+    KUN_INVALID_ARGUMENT,
+} KunStatus;
+
+typedef struct {
+    size_t version; /* must be KUN_API_VERSION */
+    KunStateBufferKind init_kind;
+    union {
+        const char *path;
+        struct {
+            const char *buffer;
+            size_t size;
+        } memory;
+    } init;
+} KunStreamExtraArgs;
+
+/* ---- reference-compatible entry points ---------------------------------- */
+
+/* CApi.h:60 kunCreateSingleThreadExecutor: an executor owns one HIP stream on
+ * the current device and a device buffer pool. */
+KUN_API KunExecutorHandle kunCreateSingleThreadExecutor(void);
+/* CApi.h:68 kunCreateMultiThreadExecutor: same device executor; numthreads
+ * bounds the host threads used for cross-checks and is otherwise ignored. */
+KUN_API KunExecutorHandle kunCreateMultiThreadExecutor(int numthreads);
+/* CApi.h:76 */
+KUN_API void kunDestoryExecutor(KunExecutorHandle ptr);
+/* CApi.h:84 kunLoadLibrary: dlopen a device module library built by
+ * kunquant_amd.jit.cfake. Returns NULL when loading fails. */
+KUN_API KunLibraryHandle kunLoadLibrary(const char *path_or_name);
+/* CApi.h:94 kunGetModuleFromLibrary: NULL when the module is not found. */
+KUN_API KunModuleHandle kunGetModuleFromLibrary(KunLibraryHandle lib, const char *name);
+/* CApi.h:102 */
+KUN_API void kunUnloadLibrary(KunLibraryHandle ptr);
+/* CApi.h:110-135 buffer name map */
+KUN_API KunBufferNameMapHandle kunCreateBufferNameMap(void);
+KUN_API void kunDestoryBufferNameMap(KunBufferNameMapHandle ptr);
+KUN_API void kunSetBufferNameMap(KunBufferNameMapHandle ptr, const char *name, float *buffer);
+KUN_API void kunEraseBufferNameMap(KunBufferNameMapHandle ptr, const char *name);
+/* CApi.h:156 kunRunGraph: computes rows [cur_time, cur_time+length) of the
+ * inputs (total_time rows each); outputs get `length` rows. Errors are
+ * reported through kunGetLastError (the reference lets exceptions escape). */
+KUN_API void kunRunGraph(KunExecutorHandle exec, KunModuleHandle m, KunBufferNameMapHandle buffers,
+                         size_t num_stocks, size_t total_time, size_t cur_time, size_t length);
+/* CApi.h:165 kunCreateStream: NULL on failure. */
+KUN_API KunStreamContextHandle kunCreateStream(KunExecutorHandle exec, KunModuleHandle m,
+                                               size_t num_stocks);
+/* CApi.h:182 kunCreateStreamEx */
+KUN_API KunStatus kunCreateStreamEx(KunExecutorHandle exec, KunModuleHandle m, size_t num_stocks,
+                                    const KunStreamExtraArgs *extra_args,
+                                    KunStreamContextHandle *out_handle);
+/* CApi.h:209 kunStreamSerializeStates: with KUN_INIT_MEMORY and a too small
+ * (or NULL) buffer, *size is set to the required size and KUN_INIT_ERROR is
+ * returned. */
+KUN_API KunStatus kunStreamSerializeStates(KunStreamContextHandle context,
+                                           KunStateBufferKind dump_kind, char *path_or_buffer,
+                                           size_t *size);
+/* CApi.h:224 */
+KUN_API size_t kunQueryBufferHandle(KunStreamContextHandle context, const char *name);
+/* CApi.h:235: host pointer to the newest row, valid until the next push/run */
+KUN_API const float *kunStreamGetCurrentBuffer(KunStreamContextHandle context, size_t handle);
+/* CApi.h:245 */
+KUN_API void kunStreamPushData(KunStreamContextHandle context, size_t handle, const float *buffer);
+/* CApi.h:253 */
+KUN_API void kunStreamRun(KunStreamContextHandle context);
+/* CApi.h:259 */
+KUN_API void kunDestoryStream(KunStreamContextHandle context);
+
+/* ---- extensions (no counterpart in CApi.h) ------------------------------ */
+
+/* Last error message of the calling thread ("" when none). */
+KUN_API const char *kunGetLastError(void);
+/* kunRunGraph with a status: 0 on success, nonzero with kunGetLastError set. */
+KUN_API int kunRunGraphEx(KunExecutorHandle exec, KunModuleHandle m, KunBufferNameMapHandle buffers,
+                          size_t num_stocks, size_t total_time, size_t cur_time, size_t length);
+/* Same computation on device-resident buffers: the name map holds device
+ * pointers in the module's layouts. Kernels are enqueued on `hip_stream`
+ * (NULL: the executor's stream); the call returns without synchronising.
+ * Temporaries come from the executor's pool, so repeated calls with the same
+ * shapes allocate nothing after the first. */
+KUN_API int kunRunGraphDevice(KunExecutorHandle exec, KunModuleHandle m, KunBufferNameMapHandle buffers,
+                              size_t num_stocks, size_t total_time, size_t cur_time, size_t length,
+                              void *hip_stream);
+/* Blocks until the executor's stream is idle. */
+KUN_API int kunExecutorSynchronize(KunExecutorHandle exec);
+
+/* Module introspection. dtype: 0 float, 1 double. layouts: 0 STs, 1 TS, 2 STREAM. */
+KUN_API int kunModuleDtype(KunModuleHandle m);
+KUN_API int kunModuleInputLayout(KunModuleHandle m);
+KUN_API int kunModuleOutputLayout(KunModuleHandle m);
+KUN_API int kunModuleBlockingLen(KunModuleHandle m);
+KUN_API int kunModuleAligned(KunModuleHandle m);
+KUN_API int kunModuleNumBuffers(KunModuleHandle m);
+/* kind: 0 input, 1 output, 2 temporary */
+KUN_API const char *kunModuleBufferName(KunModuleHandle m, int idx);
+KUN_API int kunModuleBufferKind(KunModuleHandle m, int idx);
+KUN_API int kunModuleBufferUnreliable(KunModuleHandle m, int idx);
+KUN_API int kunModuleBufferWindow(KunModuleHandle m, int idx);
+KUN_API int kunModuleNumStages(KunModuleHandle m);
+/* KUNPROG text describing the module's stage IR */
+KUN_API const char *kunModuleProgram(KunModuleHandle m);
+/* device pointer of the newest row of a stream buffer (num_stocks elements) */
+KUN_API const void *kunStreamGetCurrentDeviceBuffer(KunStreamContextHandle context, size_t handle);
+/* number of rows consumed by the stream so far */
+KUN_API size_t kunStreamTicks(KunStreamContextHandle context);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

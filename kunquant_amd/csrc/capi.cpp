@@ -1,0 +1,206 @@
+// extern "C" entry points of libkunhip.so (declared in include/kun_capi.h).
+// Handles follow the reference's conventions (cpp/Kun/CApi.cpp:11-189):
+// executors and libraries are heap-boxed shared_ptrs, module handles are
+// borrowed descriptor pointers that stay valid while the library is loaded.
+#include <string.h>
+
+#include <fstream>
+#include <iterator>
+#include <memory>
+#include <string>
+#include <unordered_map>
+
+#include "../../include/kun_capi.h"
+#include "kun_rt.hpp"
+
+using namespace kunrt;
+
+namespace {
+std::shared_ptr<Executor> *unwrap_exec(KunExecutorHandle h) { return (std::shared_ptr<Executor> *)h; }
+std::shared_ptr<Library> *unwrap_lib(KunLibraryHandle h) { return (std::shared_ptr<Library> *)h; }
+NameMap *unwrap_map(KunBufferNameMapHandle h) { return (NameMap *)h; }
+const KunModuleDesc *unwrap_mod(KunModuleHandle h) { return (const KunModuleDesc *)h; }
+
+template <typename F> int guarded(F &&f) {
+    try {
+        set_error("");
+        f();
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return 1;
+    }
+}
+} // namespace
+
+extern "C" {
+
+KUN_API const char *kunGetLastError(void) { return last_error(); }
+
+KUN_API KunExecutorHandle kunCreateSingleThreadExecutor(void) {
+    std::shared_ptr<Executor> *h = nullptr;
+    guarded([&] { h = new std::shared_ptr<Executor>(std::make_shared<Executor>()); });
+    return h;
+}
+
+KUN_API KunExecutorHandle kunCreateMultiThreadExecutor(int numthreads) {
+    (void)numthreads;
+    return kunCreateSingleThreadExecutor();
+}
+
+KUN_API void kunDestoryExecutor(KunExecutorHandle ptr) { delete unwrap_exec(ptr); }
+
+KUN_API int kunExecutorSynchronize(KunExecutorHandle exec) {
+    return guarded([&] { HIPCHECK(hipStreamSynchronize((*unwrap_exec(exec))->stream)); });
+}
+
+KUN_API KunLibraryHandle kunLoadLibrary(const char *path_or_name) {
+    auto lib = Library::load(path_or_name);
+    if (!lib) {
+        set_error(std::string("Cannot load library ") + path_or_name);
+        return nullptr;
+    }
+    return new std::shared_ptr<Library>(lib);
+}
+
+KUN_API KunModuleHandle kunGetModuleFromLibrary(KunLibraryHandle lib, const char *name) {
+    return (KunModuleHandle)(*unwrap_lib(lib))->get_module(name);
+}
+
+KUN_API void kunUnloadLibrary(KunLibraryHandle ptr) { delete unwrap_lib(ptr); }
+
+KUN_API KunBufferNameMapHandle kunCreateBufferNameMap(void) { return new NameMap(); }
+KUN_API void kunDestoryBufferNameMap(KunBufferNameMapHandle ptr) { delete unwrap_map(ptr); }
+KUN_API void kunSetBufferNameMap(KunBufferNameMapHandle ptr, const char *name, float *buffer) {
+    (*unwrap_map(ptr))[name] = buffer;
+}
+KUN_API void kunEraseBufferNameMap(KunBufferNameMapHandle ptr, const char *name) {
+    unwrap_map(ptr)->erase(name);
+}
+
+KUN_API int kunRunGraphEx(KunExecutorHandle exec, KunModuleHandle m, KunBufferNameMapHandle buffers,
+                          size_t num_stocks, size_t total_time, size_t cur_time, size_t length) {
+    return guarded([&] {
+        run_graph_host(**unwrap_exec(exec), unwrap_mod(m), *unwrap_map(buffers), num_stocks, total_time,
+                       cur_time, length);
+    });
+}
+
+KUN_API void kunRunGraph(KunExecutorHandle exec, KunModuleHandle m, KunBufferNameMapHandle buffers,
+                         size_t num_stocks, size_t total_time, size_t cur_time, size_t length) {
+    if (kunRunGraphEx(exec, m, buffers, num_stocks, total_time, cur_time, length))
+        fprintf(stderr, "kunRunGraph: %s\n", last_error());
+}
+
+KUN_API int kunRunGraphDevice(KunExecutorHandle exec, KunModuleHandle m, KunBufferNameMapHandle buffers,
+                              size_t num_stocks, size_t total_time, size_t cur_time, size_t length,
+                              void *hip_stream) {
+    return guarded([&] {
+        run_graph_device(**unwrap_exec(exec), unwrap_mod(m), *unwrap_map(buffers), num_stocks, total_time,
+                         cur_time, length, (hipStream_t)hip_stream);
+    });
+}
+
+// ---- module introspection ----------------------------------------------
+KUN_API int kunModuleDtype(KunModuleHandle m) { return unwrap_mod(m)->dtype; }
+KUN_API int kunModuleInputLayout(KunModuleHandle m) { return unwrap_mod(m)->in_layout; }
+KUN_API int kunModuleOutputLayout(KunModuleHandle m) { return unwrap_mod(m)->out_layout; }
+KUN_API int kunModuleBlockingLen(KunModuleHandle m) { return unwrap_mod(m)->blocking; }
+KUN_API int kunModuleAligned(KunModuleHandle m) { return unwrap_mod(m)->aligned; }
+KUN_API int kunModuleNumBuffers(KunModuleHandle m) { return unwrap_mod(m)->nbuf; }
+KUN_API const char *kunModuleBufferName(KunModuleHandle m, int idx) { return unwrap_mod(m)->bufs[idx].name; }
+KUN_API int kunModuleBufferKind(KunModuleHandle m, int idx) { return unwrap_mod(m)->bufs[idx].kind; }
+KUN_API int kunModuleBufferUnreliable(KunModuleHandle m, int idx) { return unwrap_mod(m)->bufs[idx].unreliable; }
+KUN_API int kunModuleBufferWindow(KunModuleHandle m, int idx) { return unwrap_mod(m)->bufs[idx].window; }
+KUN_API int kunModuleNumStages(KunModuleHandle m) { return unwrap_mod(m)->nstage; }
+KUN_API const char *kunModuleProgram(KunModuleHandle m) { return unwrap_mod(m)->program; }
+
+// ---- stream contexts ----------------------------------------------------
+KUN_API KunStatus kunCreateStreamEx(KunExecutorHandle exec, KunModuleHandle m, size_t num_stocks,
+                                    const KunStreamExtraArgs *extra_args, KunStreamContextHandle *out_handle) {
+    if (!out_handle) return KUN_INVALID_ARGUMENT;
+    *out_handle = nullptr;
+    const char *state = nullptr;
+    size_t state_len = 0;
+    bool from_file = false;
+    if (extra_args) {
+        if (extra_args->version != KUN_API_VERSION) return KUN_INVALID_ARGUMENT;
+        if (extra_args->init_kind == KUN_INIT_FILE) {
+            if (!extra_args->init.path) return KUN_INVALID_ARGUMENT;
+            state = extra_args->init.path;
+            from_file = true;
+        } else if (extra_args->init_kind == KUN_INIT_MEMORY) {
+            if (!extra_args->init.memory.buffer) return KUN_INVALID_ARGUMENT;
+            state = extra_args->init.memory.buffer;
+            state_len = extra_args->init.memory.size;
+        } else if (extra_args->init_kind != KUN_INIT_NONE) {
+            return KUN_INVALID_ARGUMENT;
+        }
+    }
+    StreamCtx *ctx = nullptr;
+    int rc = guarded([&] { ctx = stream_create(*unwrap_exec(exec), unwrap_mod(m), num_stocks, state, state_len, from_file); });
+    if (rc) return KUN_INIT_ERROR;
+    *out_handle = ctx;
+    return KUN_SUCCESS;
+}
+
+KUN_API KunStreamContextHandle kunCreateStream(KunExecutorHandle exec, KunModuleHandle m, size_t num_stocks) {
+    KunStreamContextHandle h = nullptr;
+    if (kunCreateStreamEx(exec, m, num_stocks, nullptr, &h) != KUN_SUCCESS) return nullptr;
+    return h;
+}
+
+KUN_API KunStatus kunStreamSerializeStates(KunStreamContextHandle context, KunStateBufferKind dump_kind,
+                                           char *path_or_buffer, size_t *size) {
+    if (dump_kind != KUN_INIT_FILE && dump_kind != KUN_INIT_MEMORY) return KUN_INVALID_ARGUMENT;
+    std::string blob;
+    if (guarded([&] { blob = stream_serialize((StreamCtx *)context); })) return KUN_INIT_ERROR;
+    if (dump_kind == KUN_INIT_FILE) {
+        if (!path_or_buffer) return KUN_INVALID_ARGUMENT;
+        std::ofstream f(path_or_buffer, std::ios::binary);
+        if (!f) return KUN_INIT_ERROR;
+        f.write(blob.data(), (std::streamsize)blob.size());
+        return f ? KUN_SUCCESS : KUN_INIT_ERROR;
+    }
+    if (!size) return KUN_INVALID_ARGUMENT;
+    if (!path_or_buffer || *size < blob.size()) {
+        *size = blob.size();
+        return KUN_INIT_ERROR;
+    }
+    memcpy(path_or_buffer, blob.data(), blob.size());
+    *size = blob.size();
+    return KUN_SUCCESS;
+}
+
+KUN_API size_t kunQueryBufferHandle(KunStreamContextHandle context, const char *name) {
+    size_t h = (size_t)-1;
+    guarded([&] { h = stream_query((StreamCtx *)context, name); });
+    return h;
+}
+
+KUN_API const float *kunStreamGetCurrentBuffer(KunStreamContextHandle context, size_t handle) {
+    const void *p = nullptr;
+    guarded([&] { p = stream_current((StreamCtx *)context, handle); });
+    return (const float *)p;
+}
+
+KUN_API const void *kunStreamGetCurrentDeviceBuffer(KunStreamContextHandle context, size_t handle) {
+    const void *p = nullptr;
+    guarded([&] { p = stream_current_device((StreamCtx *)context, handle); });
+    return p;
+}
+
+KUN_API void kunStreamPushData(KunStreamContextHandle context, size_t handle, const float *buffer) {
+    if (guarded([&] { stream_push((StreamCtx *)context, handle, buffer); }))
+        fprintf(stderr, "kunStreamPushData: %s\n", last_error());
+}
+
+KUN_API void kunStreamRun(KunStreamContextHandle context) {
+    if (guarded([&] { stream_run((StreamCtx *)context); })) fprintf(stderr, "kunStreamRun: %s\n", last_error());
+}
+
+KUN_API size_t kunStreamTicks(KunStreamContextHandle context) { return stream_ticks((StreamCtx *)context); }
+
+KUN_API void kunDestoryStream(KunStreamContextHandle context) { stream_destroy((StreamCtx *)context); }
+
+} // extern "C"

@@ -1,0 +1,57 @@
+/* Descriptor ABI between generated device modules and the device runtime.
+ *
+ * A compiled module library exports one `const KunModuleDesc kun_module_<name>`
+ * per module; the runtime finds it with dlsym (the MI355X counterpart of the
+ * reference's `KUN_EXPORT Module <name>` symbol, Driver.py:324-337,
+ * Module.cpp:36-38). Stage kernels are referenced by their host stubs and
+ * launched through hipLaunchKernel. */
+#ifndef KUN_MODULE_ABI_H
+#define KUN_MODULE_ABI_H
+
+#define KUN_MODULE_ABI_VERSION 0x4B554E02u
+
+enum { KUN_BUF_INPUT = 0, KUN_BUF_OUTPUT = 1, KUN_BUF_TEMP = 2 };
+enum { KUN_STAGE_TS = 0, KUN_STAGE_RANK = 1, KUN_STAGE_SCALE = 2 };
+enum { KUN_LAYOUT_STS = 0, KUN_LAYOUT_TS = 1, KUN_LAYOUT_STREAM = 2 };
+enum { KUN_DTYPE_FLOAT = 0, KUN_DTYPE_DOUBLE = 1 };
+
+typedef struct {
+    const char *name;
+    int kind;       /* KUN_BUF_* */
+    int unreliable; /* warm-up rows of an output */
+    int window;     /* rows of history consumers need */
+} KunBufferDesc;
+
+typedef struct {
+    const char *name;
+    int kind;             /* KUN_STAGE_* */
+    const void *kernel;   /* batch kernel host stub (TS stages) */
+    const void *stream_kernel; /* one-tick kernel host stub (STREAM modules) */
+    int block;            /* threads per workgroup */
+    int lds_bytes;        /* static LDS per workgroup */
+    int state_bytes;      /* per-stock HBM bytes: carried state (STREAM) or spilled rings (batch) */
+    const int *in;
+    int nin;
+    const int *out;
+    int nout;
+    const int *deps;
+    int ndeps;
+} KunStageDesc;
+
+typedef struct {
+    unsigned abi;
+    const char *name;
+    int dtype;      /* KUN_DTYPE_* */
+    int in_layout;  /* KUN_LAYOUT_* */
+    int out_layout;
+    int blocking;   /* STs blocking length */
+    int aligned;    /* stock count must be a multiple of blocking */
+    int nbuf;
+    const KunBufferDesc *bufs;
+    int nstage;
+    const KunStageDesc *stages;
+    const char *program; /* KUNPROG text of the module */
+    const void *const *stream_inits; /* per stage: state-initialising kernel (STREAM modules) */
+} KunModuleDesc;
+
+#endif

@@ -1,0 +1,85 @@
+// Internal interfaces of the device runtime (runtime.hip, stream.hip, capi.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <unordered_map>
+#include <vector>
+
+#include "kun_module_abi.h"
+
+namespace kunrt {
+
+struct Error : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define HIPCHECK(x)                                                                              \
+    do {                                                                                         \
+        hipError_t e_ = (x);                                                                     \
+        if (e_ != hipSuccess) throw ::kunrt::Error(std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+void set_error(const std::string &msg);
+const char *last_error();
+
+using NameMap = std::unordered_map<std::string, void *>;
+
+// exact-size caching device allocator
+struct Pool {
+    std::mutex mu;
+    std::multimap<size_t, void *> free_;
+    void *get(size_t bytes);
+    void put(void *p, size_t bytes);
+    ~Pool();
+};
+
+// An executor: one device, one HIP stream, a buffer pool.
+struct Executor {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Pool pool;
+    std::mutex mu;
+    // temporaries of device-resident runs, per (module, stocks, rows)
+    std::map<std::tuple<const void *, size_t, size_t>, std::vector<void *>> temps;
+    Executor();
+    ~Executor();
+    void free_temps();
+};
+
+struct Library {
+    void *handle = nullptr;
+    std::string path;
+    static std::shared_ptr<Library> load(const char *path);
+    const KunModuleDesc *get_module(const char *name);
+    ~Library();
+};
+
+std::string module_symbol(const std::string &name);
+const std::vector<int> &stage_order(const KunModuleDesc *m);
+size_t elem_size(const KunModuleDesc *m);
+size_t buffer_elems(const KunModuleDesc *m, int b, size_t S, size_t T, size_t L);
+void run_graph_host(Executor &ex, const KunModuleDesc *m, const NameMap &bufs, size_t S, size_t T,
+                    size_t start, size_t L);
+void run_graph_device(Executor &ex, const KunModuleDesc *m, const NameMap &bufs, size_t S, size_t T,
+                      size_t start, size_t L, hipStream_t st);
+
+// ---- stream contexts ----------------------------------------------------
+struct StreamCtx;
+StreamCtx *stream_create(std::shared_ptr<Executor> ex, const KunModuleDesc *m, size_t S,
+                         const char *state, size_t state_len, bool from_file);
+void stream_destroy(StreamCtx *c);
+size_t stream_query(StreamCtx *c, const char *name);
+void stream_push(StreamCtx *c, size_t handle, const void *host_row);
+void stream_run(StreamCtx *c);
+const void *stream_current(StreamCtx *c, size_t handle);
+const void *stream_current_device(StreamCtx *c, size_t handle);
+size_t stream_ticks(StreamCtx *c);
+std::string stream_serialize(StreamCtx *c);
+
+} // namespace kunrt

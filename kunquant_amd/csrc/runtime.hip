@@ -1,0 +1,306 @@
+// Device runtime of the factor pipeline: executors, module libraries, batch
+// graph execution and stream contexts, exported through the C ABI of
+// include/kun_capi.h.
+//
+// Replaces the reference's CPU runtime (cpp/Kun/Runtime.cpp, Executor.cpp,
+// Module.cpp, CApi.cpp): instead of a thread pool claiming 8-stock tasks, a
+// stage is one kernel launch over all stocks (TS stages: 64 stocks per
+// workgroup; cross-sectional stages: one workgroup per row), enqueued in a
+// topological order of the stage DAG on the executor's HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <dlfcn.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/kun_capi.h"
+#include "kun_cs.hpp"
+#include "kun_device.hpp"
+#include "kun_module_abi.h"
+#include "kun_rt.hpp"
+
+namespace kunrt {
+
+thread_local std::string g_last_error;
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+const char *last_error() { return g_last_error.c_str(); }
+
+// ---------------------------------------------------------------------------
+// device buffer pool: exact-size caching allocator (hipMalloc only on miss)
+void *Pool::get(size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    std::lock_guard<std::mutex> g(mu);
+    auto it = free_.find(bytes);
+    if (it != free_.end()) {
+        void *p = it->second;
+        free_.erase(it);
+        return p;
+    }
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        // release cached blocks and retry once
+        for (auto &kv : free_) hipFree(kv.second);
+        free_.clear();
+        e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) throw Error(std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    }
+    return p;
+}
+
+void Pool::put(void *p, size_t bytes) {
+    if (!p) return;
+    if (bytes == 0) bytes = 256;
+    std::lock_guard<std::mutex> g(mu);
+    free_.emplace(bytes, p);
+}
+
+Pool::~Pool() {
+    for (auto &kv : free_) hipFree(kv.second);
+}
+
+Executor::Executor() {
+    HIPCHECK(hipGetDevice(&device));
+    HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+}
+
+Executor::~Executor() {
+    hipStreamSynchronize(stream);
+    hipStreamDestroy(stream);
+}
+
+// ---------------------------------------------------------------------------
+std::string module_symbol(const std::string &name) {
+    std::string s = "kun_module_";
+    for (char c : name) s += (isalnum((unsigned char)c) ? c : '_');
+    return s;
+}
+
+Library::~Library() {
+    if (handle) dlclose(handle);
+}
+
+std::shared_ptr<Library> Library::load(const char *path) {
+    void *h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
+    if (!h) {
+        fprintf(stderr, "dlopen failed: %s %s\n", path, dlerror());
+        return nullptr;
+    }
+    auto lib = std::make_shared<Library>();
+    lib->handle = h;
+    lib->path = path;
+    return lib;
+}
+
+const KunModuleDesc *Library::get_module(const char *name) {
+    auto *m = (const KunModuleDesc *)dlsym(handle, module_symbol(name).c_str());
+    if (m && m->abi != KUN_MODULE_ABI_VERSION) return nullptr;
+    return m;
+}
+
+// topological order of the stage DAG (cached per descriptor)
+static std::mutex g_order_mu;
+static std::unordered_map<const KunModuleDesc *, std::vector<int>> g_orders;
+
+const std::vector<int> &stage_order(const KunModuleDesc *m) {
+    std::lock_guard<std::mutex> g(g_order_mu);
+    auto it = g_orders.find(m);
+    if (it != g_orders.end()) return it->second;
+    std::vector<int> indeg(m->nstage), order;
+    for (int i = 0; i < m->nstage; i++) indeg[i] = m->stages[i].ndeps;
+    for (int i = 0; i < m->nstage; i++)
+        if (!indeg[i]) order.push_back(i);
+    for (size_t h = 0; h < order.size(); h++) {
+        int s = order[h];
+        for (int j = 0; j < m->nstage; j++)
+            for (int d = 0; d < m->stages[j].ndeps; d++)
+                if (m->stages[j].deps[d] == s && --indeg[j] == 0) order.push_back(j);
+    }
+    if ((int)order.size() != m->nstage) throw Error("stage graph of the module has a cycle");
+    return g_orders.emplace(m, std::move(order)).first->second;
+}
+
+static bool is_single_value(const KunBufferDesc &b) { return strncmp(b.name, "__init", 6) == 0; }
+
+size_t elem_size(const KunModuleDesc *m) { return m->dtype == KUN_DTYPE_DOUBLE ? 8 : 4; }
+
+// elements a buffer holds for a batch run
+size_t buffer_elems(const KunModuleDesc *m, int b, size_t S, size_t T, size_t L) {
+    const KunBufferDesc &d = m->bufs[b];
+    size_t blk = (size_t)m->blocking;
+    size_t Sr = (S + blk - 1) / blk * blk;
+    if (d.kind == KUN_BUF_INPUT) {
+        if (is_single_value(d)) return S;
+        return (m->in_layout == KUN_LAYOUT_STS ? Sr : S) * T;
+    }
+    if (d.kind == KUN_BUF_OUTPUT) return (m->out_layout == KUN_LAYOUT_STS ? Sr : S) * L;
+    return S * L;
+}
+
+static kun_cs::View cs_view(const KunModuleDesc *m, int b, void *ptr, size_t T, size_t start, size_t L) {
+    const KunBufferDesc &d = m->bufs[b];
+    kun_cs::View v;
+    v.ptr = ptr;
+    if (d.kind == KUN_BUF_INPUT) {
+        v.layout = m->in_layout == KUN_LAYOUT_STS ? 0 : 1;
+        v.rows = (long)T;
+        v.row0 = (long)start;
+    } else {
+        v.layout = (d.kind == KUN_BUF_OUTPUT && m->out_layout == KUN_LAYOUT_STS) ? 0 : 1;
+        v.rows = (long)L;
+        v.row0 = 0;
+    }
+    return v;
+}
+
+// Enqueue every stage of a batch module; dev[b] are device pointers,
+// scratch[s] the HBM scratch of stage s (or null).
+void enqueue_batch(const KunModuleDesc *m, const std::vector<void *> &dev, const std::vector<void *> &scratch,
+                   size_t S, size_t T, size_t start, size_t L, hipStream_t st) {
+    if (L == 0) return;
+    for (int si : stage_order(m)) {
+        const KunStageDesc &sd = m->stages[si];
+        if (sd.kind == KUN_STAGE_TS) {
+            kun::StageArgs a;
+            memset(&a, 0, sizeof a);
+            int n = 0;
+            for (int j = 0; j < sd.nin; j++) a.bufs[n++] = dev[sd.in[j]];
+            for (int j = 0; j < sd.nout; j++) a.bufs[n++] = dev[sd.out[j]];
+            a.S = (long)S;
+            a.T = (long)T;
+            a.start = (long)start;
+            a.L = (long)L;
+            a.blk = m->blocking;
+            a.scratch = scratch[si];
+            void *args[] = {&a};
+            unsigned grid = (unsigned)((S + sd.block - 1) / sd.block);
+            HIPCHECK(hipLaunchKernel(sd.kernel, dim3(grid), dim3(sd.block), args, 0, st));
+        } else {
+            int ib = sd.in[0], ob = sd.out[0];
+            kun_cs::View vi = cs_view(m, ib, dev[ib], T, start, L);
+            kun_cs::View vo = cs_view(m, ob, dev[ob], T, start, L);
+            if (sd.kind == KUN_STAGE_RANK && (long)S > kun_cs::max_rank_stocks(m->dtype))
+                throw Error("Rank over more than " + std::to_string(kun_cs::max_rank_stocks(m->dtype)) +
+                            " stocks is not supported");
+            hipError_t e = sd.kind == KUN_STAGE_RANK
+                               ? kun_cs::launch_rank(m->dtype, vi, vo, (long)S, (long)L, m->blocking, st)
+                               : kun_cs::launch_scale(m->dtype, vi, vo, (long)S, (long)L, m->blocking, st);
+            if (e != hipSuccess) throw Error(std::string("cross-sectional kernel: ") + hipGetErrorString(e));
+        }
+    }
+}
+
+void check_batch_module(const KunModuleDesc *m, size_t S, size_t T, size_t start, size_t L) {
+    if (!m) throw Error("null module");
+    if (m->abi != KUN_MODULE_ABI_VERSION)
+        throw Error("The required version in the module does not match the runtime version");
+    if (m->out_layout == KUN_LAYOUT_STREAM) throw Error("Cannot run stream mode module via runGraph()");
+    if (start + L > T) throw Error("Bad parameter: cur_time + length exceeds total_time");
+    if (m->aligned && S % (size_t)m->blocking) throw Error("num_stocks must be a multiple of blocking_len");
+}
+
+// Host-buffer run: H2D inputs, enqueue, D2H outputs, synchronise.
+void run_graph_host(Executor &ex, const KunModuleDesc *m, const NameMap &bufs, size_t S, size_t T,
+                    size_t start, size_t L) {
+    check_batch_module(m, S, T, start, L);
+    std::lock_guard<std::mutex> g(ex.mu);
+    HIPCHECK(hipSetDevice(ex.device));
+    const size_t esz = elem_size(m);
+    std::vector<void *> dev(m->nbuf, nullptr);
+    std::vector<size_t> bytes(m->nbuf, 0);
+    struct Release {
+        Executor &ex;
+        std::vector<void *> &dev;
+        std::vector<size_t> &bytes;
+        ~Release() {
+            hipStreamSynchronize(ex.stream);
+            for (size_t i = 0; i < dev.size(); i++) ex.pool.put(dev[i], bytes[i]);
+        }
+    } release{ex, dev, bytes};
+    for (int b = 0; b < m->nbuf; b++) {
+        const KunBufferDesc &d = m->bufs[b];
+        bytes[b] = buffer_elems(m, b, S, T, L) * esz;
+        if (d.kind != KUN_BUF_TEMP && !bufs.count(d.name))
+            throw Error(std::string("Buffer name not found: ") + d.name);
+        dev[b] = ex.pool.get(bytes[b]);
+        if (d.kind == KUN_BUF_INPUT)
+            HIPCHECK(hipMemcpyAsync(dev[b], bufs.at(d.name), bytes[b], hipMemcpyHostToDevice, ex.stream));
+    }
+    std::vector<void *> scratch(m->nstage, nullptr);
+    struct ReleaseScratch {
+        Executor &ex;
+        const KunModuleDesc *m;
+        std::vector<void *> &scratch;
+        size_t S;
+        ~ReleaseScratch() {
+            hipStreamSynchronize(ex.stream);
+            for (int i = 0; i < m->nstage; i++) ex.pool.put(scratch[i], (size_t)m->stages[i].state_bytes * S);
+        }
+    } release_scratch{ex, m, scratch, S};
+    for (int i = 0; i < m->nstage; i++)
+        if (m->stages[i].kind == KUN_STAGE_TS && m->stages[i].state_bytes)
+            scratch[i] = ex.pool.get((size_t)m->stages[i].state_bytes * S);
+    enqueue_batch(m, dev, scratch, S, T, start, L, ex.stream);
+    for (int b = 0; b < m->nbuf; b++) {
+        const KunBufferDesc &d = m->bufs[b];
+        if (d.kind == KUN_BUF_OUTPUT)
+            HIPCHECK(hipMemcpyAsync(bufs.at(d.name), dev[b], bytes[b], hipMemcpyDeviceToHost, ex.stream));
+    }
+    HIPCHECK(hipStreamSynchronize(ex.stream));
+}
+
+// Device-buffer run: temporaries from the pool (kept for reuse per shape).
+void run_graph_device(Executor &ex, const KunModuleDesc *m, const NameMap &bufs, size_t S, size_t T,
+                      size_t start, size_t L, hipStream_t st) {
+    check_batch_module(m, S, T, start, L);
+    std::lock_guard<std::mutex> g(ex.mu);
+    HIPCHECK(hipSetDevice(ex.device));
+    if (!st) st = ex.stream;
+    const size_t esz = elem_size(m);
+    auto key = std::make_tuple((const void *)m, S, L);
+    auto &temps = ex.temps[key];
+    if (temps.empty()) {
+        // buffers first, then one scratch area per stage
+        temps.resize(m->nbuf + m->nstage, nullptr);
+        for (int b = 0; b < m->nbuf; b++)
+            if (m->bufs[b].kind == KUN_BUF_TEMP) {
+                size_t nb = buffer_elems(m, b, S, T, L) * esz;
+                HIPCHECK(hipMalloc(&temps[b], nb ? nb : 256));
+            }
+        for (int i = 0; i < m->nstage; i++)
+            if (m->stages[i].kind == KUN_STAGE_TS && m->stages[i].state_bytes)
+                HIPCHECK(hipMalloc(&temps[m->nbuf + i], (size_t)m->stages[i].state_bytes * S));
+    }
+    std::vector<void *> scratch(temps.begin() + m->nbuf, temps.end());
+    std::vector<void *> dev(m->nbuf, nullptr);
+    for (int b = 0; b < m->nbuf; b++) {
+        const KunBufferDesc &d = m->bufs[b];
+        if (d.kind == KUN_BUF_TEMP) {
+            dev[b] = temps[b];
+        } else {
+            auto it = bufs.find(d.name);
+            if (it == bufs.end()) throw Error(std::string("Buffer name not found: ") + d.name);
+            dev[b] = it->second;
+        }
+    }
+    enqueue_batch(m, dev, scratch, S, T, start, L, st);
+}
+
+void Executor::free_temps() {
+    for (auto &kv : temps)
+        for (void *p : kv.second)
+            if (p) hipFree(p);
+    temps.clear();
+}
+
+} // namespace kunrt

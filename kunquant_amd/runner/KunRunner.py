@@ -1,0 +1,397 @@
+"""KunRunner: the Python surface of the reference's pybind11 module
+(cpp/Python/PyBinding.cpp:146-597), implemented over the C ABI of
+libkunhip.so (include/kun_capi.h) with ctypes.
+
+Same names, argument meaning, shape/dtype validation and error messages as
+the reference: runGraph validates exactly like PyBinding.cpp:242-394 and
+raises RuntimeError; StreamContext mirrors PyBinding.cpp:508-596."""
+import ctypes
+import os
+from typing import Dict, Optional
+
+import numpy as np
+
+from .. import PACKAGE_DIR
+
+_LIB_PATH = os.path.join(PACKAGE_DIR, "lib", "libkunhip.so")
+_lib = None
+
+_LAYOUTS = {0: "STs", 1: "TS", 2: "STREAM"}
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise RuntimeError(f"libkunhip.so is not built ({_LIB_PATH}); run __graft_entry__.build() "
+                           f"or kunquant_amd.build.build_runtime()")
+    L = ctypes.CDLL(_LIB_PATH, mode=ctypes.RTLD_GLOBAL)
+    vp, sz, cp, i = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_char_p, ctypes.c_int
+
+    def sig(name, res, *args):
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = list(args)
+
+    sig("kunGetLastError", cp)
+    sig("kunCreateSingleThreadExecutor", vp)
+    sig("kunCreateMultiThreadExecutor", vp, i)
+    sig("kunDestoryExecutor", None, vp)
+    sig("kunExecutorSynchronize", i, vp)
+    sig("kunLoadLibrary", vp, cp)
+    sig("kunGetModuleFromLibrary", vp, vp, cp)
+    sig("kunUnloadLibrary", None, vp)
+    sig("kunCreateBufferNameMap", vp)
+    sig("kunDestoryBufferNameMap", None, vp)
+    sig("kunSetBufferNameMap", None, vp, cp, vp)
+    sig("kunEraseBufferNameMap", None, vp, cp)
+    sig("kunRunGraphEx", i, vp, vp, vp, sz, sz, sz, sz)
+    sig("kunRunGraphDevice", i, vp, vp, vp, sz, sz, sz, sz, vp)
+    for n in ("kunModuleDtype", "kunModuleInputLayout", "kunModuleOutputLayout", "kunModuleBlockingLen",
+              "kunModuleAligned", "kunModuleNumBuffers", "kunModuleNumStages"):
+        sig(n, i, vp)
+    sig("kunModuleBufferName", cp, vp, i)
+    for n in ("kunModuleBufferKind", "kunModuleBufferUnreliable", "kunModuleBufferWindow"):
+        sig(n, i, vp, i)
+    sig("kunModuleProgram", cp, vp)
+    sig("kunCreateStreamEx", i, vp, vp, sz, vp, ctypes.POINTER(vp))
+    sig("kunStreamSerializeStates", i, vp, i, vp, ctypes.POINTER(sz))
+    sig("kunQueryBufferHandle", sz, vp, cp)
+    sig("kunStreamGetCurrentBuffer", vp, vp, sz)
+    sig("kunStreamGetCurrentDeviceBuffer", vp, vp, sz)
+    sig("kunStreamPushData", None, vp, sz, vp)
+    sig("kunStreamRun", None, vp)
+    sig("kunStreamTicks", sz, vp)
+    sig("kunDestoryStream", None, vp)
+    _lib = L
+    return L
+
+
+def _err() -> str:
+    return _load().kunGetLastError().decode()
+
+
+def getRuntimePath() -> str:
+    return _LIB_PATH
+
+
+class Executor:
+    """A device executor (one HIP stream on the current device)."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.kunDestoryExecutor(self._h)
+            self._h = None
+
+    def synchronize(self):
+        if _load().kunExecutorSynchronize(self._h):
+            raise RuntimeError(_err())
+
+
+def createSingleThreadExecutor() -> Executor:
+    h = _load().kunCreateSingleThreadExecutor()
+    if not h:
+        raise RuntimeError(_err())
+    return Executor(h)
+
+
+def createMultiThreadExecutor(num_threads: int) -> Executor:
+    h = _load().kunCreateMultiThreadExecutor(int(num_threads))
+    if not h:
+        raise RuntimeError(_err())
+    return Executor(h)
+
+
+class Module:
+    def __init__(self, handle, lib: "Library"):
+        self._h = handle
+        self._lib = lib  # keeps the library loaded
+        L = _load()
+        self.dtype = np.float64 if L.kunModuleDtype(handle) == 1 else np.float32
+        self.input_layout = _LAYOUTS[L.kunModuleInputLayout(handle)]
+        self._output_layout = _LAYOUTS[L.kunModuleOutputLayout(handle)]
+        self._blocking = L.kunModuleBlockingLen(handle)
+        self.aligned = bool(L.kunModuleAligned(handle))
+        self.buffers = []
+        for b in range(L.kunModuleNumBuffers(handle)):
+            self.buffers.append((L.kunModuleBufferName(handle, b).decode(), L.kunModuleBufferKind(handle, b),
+                                 L.kunModuleBufferUnreliable(handle, b), L.kunModuleBufferWindow(handle, b)))
+
+    @property
+    def output_layout(self) -> str:
+        return self._output_layout
+
+    @property
+    def blocking_len(self) -> int:
+        return self._blocking
+
+    def getOutputNames(self):
+        return [n for n, k, _, _ in self.buffers if k == 1]
+
+    def getInputNames(self):
+        return [n for n, k, _, _ in self.buffers if k == 0]
+
+    def getOutputUnreliableCount(self) -> Dict[str, int]:
+        return {n: u for n, k, u, _ in self.buffers if k == 1}
+
+    def program(self) -> str:
+        return _load().kunModuleProgram(self._h).decode()
+
+
+class Library:
+    def __init__(self, handle, path: str):
+        self._h = handle
+        self.path = path
+        self._cleanup = None
+
+    @staticmethod
+    def load(filename: str) -> "Library":
+        h = _load().kunLoadLibrary(filename.encode())
+        if not h:
+            raise RuntimeError("Cannot load library")
+        return Library(h, filename)
+
+    def setCleanup(self, fn):
+        self._cleanup = fn
+
+    def getModule(self, name: str) -> Module:
+        h = _load().kunGetModuleFromLibrary(self._h, name.encode())
+        if not h:
+            raise RuntimeError("Module name not found")
+        return Module(h, self)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.kunUnloadLibrary(self._h)
+            self._h = None
+            if self._cleanup:
+                self._cleanup()
+
+
+def _shape_str(shape) -> str:
+    return "(" + ", ".join(str(int(x)) for x in shape) + ")"
+
+
+def _expect(dtype, arr: np.ndarray, name: str, shape):
+    if arr.dtype != dtype:
+        raise RuntimeError(("Expecting float buffer at " if dtype == np.float32 else "Expecting double buffer at ")
+                           + name)
+    if arr.ndim != len(shape) or tuple(arr.shape) != tuple(shape):
+        raise RuntimeError(f"Bad shape at {name} expected {_shape_str(shape)} but got {_shape_str(arr.shape)}")
+    if any(s <= 0 for s in arr.shape):
+        raise RuntimeError("Bad dimension number at " + name)
+    if not arr.flags.c_contiguous:
+        raise RuntimeError("Bad stride at " + name)
+
+
+def _name_map(bufs: Dict[str, int]):
+    L = _load()
+    h = L.kunCreateBufferNameMap()
+    for k, v in bufs.items():
+        L.kunSetBufferNameMap(h, k.encode(), ctypes.c_void_p(v))
+    return h
+
+
+def runGraph(exec: Executor, mod: Module, inputs: dict, cur_time: int, length: int, outputs: dict = None,
+             skip_check: bool = False, num_stocks: int = -1) -> dict:
+    """Batch execution (PyBinding.cpp:242-394): copies inputs to HBM, runs
+    every stage on the device and returns the outputs as numpy arrays."""
+    if outputs is None:
+        outputs = {}
+    dtype = mod.dtype
+    simd_len = mod.blocking_len
+    known_S = known_T = known_num_stocks = 0
+    bufs: Dict[str, int] = {}
+    keep = []
+    for name, arr in inputs.items():
+        if not isinstance(arr, np.ndarray):
+            arr = np.asarray(arr)
+        keep.append(arr)
+        if skip_check:
+            if known_S == 0 and not name.startswith("__init"):
+                if mod.input_layout == "STs":
+                    known_S, known_T = arr.shape[0], arr.shape[1]
+                elif mod.input_layout == "TS":
+                    known_S, known_T = arr.shape[1] // simd_len, arr.shape[0]
+                    known_num_stocks = arr.shape[1]
+            bufs[name] = arr.ctypes.data
+            continue
+        if arr.dtype != dtype:
+            raise RuntimeError(("Expecting float buffer at " if dtype == np.float32 else
+                                "Expecting double buffer at ") + name)
+        if name.startswith("__init"):
+            if arr.ndim != 1:
+                raise RuntimeError("Bad Init shape at " + name)
+            if not known_num_stocks:
+                known_num_stocks = arr.shape[0]
+            _expect(dtype, arr, name, (known_num_stocks,))
+        elif mod.input_layout == "STs":
+            if arr.ndim != 3:
+                raise RuntimeError("Bad STs shape at " + name)
+            if known_S == 0:
+                known_S, known_T = arr.shape[0], arr.shape[1]
+                if not known_num_stocks:
+                    known_num_stocks = known_S * simd_len
+            _expect(dtype, arr, name, (known_S, known_T, simd_len))
+        elif mod.input_layout == "TS":
+            if arr.ndim != 2:
+                raise RuntimeError("Bad TS shape at " + name)
+            if known_S == 0:
+                known_S, known_T = arr.shape[1] // simd_len, arr.shape[0]
+                if not known_num_stocks:
+                    known_num_stocks = arr.shape[1]
+                if mod.aligned and known_num_stocks % simd_len != 0:
+                    raise RuntimeError("Bad shape at " + name)
+            _expect(dtype, arr, name, (known_T, known_num_stocks))
+        else:
+            raise RuntimeError("Unknown layout at " + name)
+        bufs[name] = arr.ctypes.data
+    if num_stocks < 0:
+        num_stocks = known_num_stocks
+    if not skip_check:
+        if length > known_T:
+            raise RuntimeError("Bad parameter: length")
+        if mod.input_layout == "STs":
+            if num_stocks > known_num_stocks or known_num_stocks <= known_num_stocks - simd_len:
+                raise RuntimeError("num_stocks does not match the shape of inputs")
+        elif num_stocks != known_num_stocks:
+            raise RuntimeError("num_stocks does not match the shape of inputs")
+    if mod.output_layout == "STs":
+        out_shape = (known_S, length, simd_len)
+    else:
+        out_shape = (length, num_stocks)
+    ret = {}
+    for name in mod.getOutputNames():
+        if name in outputs:
+            arr = outputs[name]
+            if not skip_check:
+                _expect(dtype, arr, name, out_shape)
+        else:
+            arr = np.empty(out_shape, dtype=dtype)
+        bufs[name] = arr.ctypes.data
+        ret[name] = arr
+    L = _load()
+    h = _name_map(bufs)
+    try:
+        rc = L.kunRunGraphEx(exec._h, mod._h, h, num_stocks, known_T, cur_time, length)
+    finally:
+        L.kunDestoryBufferNameMap(h)
+    if rc:
+        raise RuntimeError(_err())
+    return ret
+
+
+def runGraphDevice(exec: Executor, mod: Module, buffers: Dict[str, int], num_stocks: int, total_time: int,
+                   cur_time: int, length: int, stream: int = 0):
+    """Device-resident execution: `buffers` maps every input and output name
+    to a device pointer (e.g. torch.Tensor.data_ptr()) in the module's
+    layouts. Enqueues on `stream` (0: the executor's stream) and returns
+    without synchronising."""
+    L = _load()
+    h = _name_map(buffers)
+    try:
+        rc = L.kunRunGraphDevice(exec._h, mod._h, h, num_stocks, total_time, cur_time, length,
+                                 ctypes.c_void_p(stream) if stream else None)
+    finally:
+        L.kunDestoryBufferNameMap(h)
+    if rc:
+        raise RuntimeError(_err())
+
+
+class StreamContext:
+    """Tick-by-tick execution (PyBinding.cpp:508-596); state is resident in
+    HBM between ticks."""
+
+    def __init__(self, exec: Executor, mod: Module, stocks: int, init=None):
+        L = _load()
+        self._mod = mod
+        self._exec = exec
+        self._S = int(stocks)
+        out = ctypes.c_void_p()
+        if init is None:
+            rc = L.kunCreateStreamEx(exec._h, mod._h, self._S, None, ctypes.byref(out))
+        else:
+            args = _StreamExtraArgs()
+            args.version = 1
+            if isinstance(init, str):
+                args.init_kind = 1
+                self._init_keep = init.encode()
+                args.init.path = self._init_keep
+            elif isinstance(init, (bytes, bytearray)):
+                args.init_kind = 2
+                self._init_keep = bytes(init)
+                args.init.memory.buffer = self._init_keep
+                args.init.memory.size = len(self._init_keep)
+            else:
+                raise RuntimeError("Bad type for init, expecting filename or bytes")
+            rc = L.kunCreateStreamEx(exec._h, mod._h, self._S, ctypes.byref(args), ctypes.byref(out))
+        if rc:
+            raise RuntimeError(_err() or "Failed to create stream")
+        self._h = out.value
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.kunDestoryStream(self._h)
+            self._h = None
+
+    def queryBufferHandle(self, name: str) -> int:
+        h = _load().kunQueryBufferHandle(self._h, name.encode())
+        if h == ctypes.c_size_t(-1).value:
+            raise RuntimeError(_err())
+        return h
+
+    def getCurrentBuffer(self, handle: int) -> np.ndarray:
+        ptr = _load().kunStreamGetCurrentBuffer(self._h, handle)
+        if not ptr:
+            raise RuntimeError(_err())
+        ct = ctypes.c_double if self._mod.dtype == np.float64 else ctypes.c_float
+        return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ct)), shape=(self._S,))
+
+    def getCurrentDeviceBuffer(self, handle: int) -> int:
+        return _load().kunStreamGetCurrentDeviceBuffer(self._h, handle)
+
+    def pushData(self, handle: int, data: np.ndarray):
+        dtype = self._mod.dtype
+        if not isinstance(data, np.ndarray) or data.dtype != dtype or not data.flags.c_contiguous:
+            raise RuntimeError("Bad input type to push, expecting float")
+        if data.ndim != 1 or data.shape[0] != self._S:
+            raise RuntimeError("Bad dimension for input data to push")
+        _load().kunStreamPushData(self._h, handle, data.ctypes.data)
+
+    def run(self):
+        _load().kunStreamRun(self._h)
+        e = _err()
+        if e:
+            raise RuntimeError(e)
+
+    def serializeStates(self, fileNameOrNone=None):
+        L = _load()
+        if isinstance(fileNameOrNone, str):
+            size = ctypes.c_size_t(0)
+            rc = L.kunStreamSerializeStates(self._h, 1, ctypes.c_char_p(fileNameOrNone.encode()), ctypes.byref(size))
+            if rc:
+                raise RuntimeError("Failed to serialize states")
+            return None
+        size = ctypes.c_size_t(0)
+        L.kunStreamSerializeStates(self._h, 2, None, ctypes.byref(size))
+        buf = ctypes.create_string_buffer(size.value)
+        rc = L.kunStreamSerializeStates(self._h, 2, buf, ctypes.byref(size))
+        if rc:
+            raise RuntimeError("Failed to serialize states")
+        return buf.raw[:size.value]
+
+
+class _Mem(ctypes.Structure):
+    _fields_ = [("buffer", ctypes.c_char_p), ("size", ctypes.c_size_t)]
+
+
+class _Init(ctypes.Union):
+    _fields_ = [("path", ctypes.c_char_p), ("memory", _Mem)]
+
+
+class _StreamExtraArgs(ctypes.Structure):
+    _fields_ = [("version", ctypes.c_size_t), ("init_kind", ctypes.c_int), ("init", _Init)]

@@ -1,6 +1,11 @@
-"""Factor graphs of the benchmark workloads.
+"""Factor graphs of the benchmark workloads and the test catalog.
 
-simple_graph(): x = (close - open) / ((high - low) + 0.001), WindowedStddev(x, 20),
+Every builder takes a namespace `K` of graph constructors (Builder, Input,
+Output, op classes). By default it is this package's front-end; the golden
+vector script passes the reference's classes so that both compilers see the
+same graph.
+
+simple_graph: x = (close - open) / ((high - low) + 0.001), WindowedStddev(x, 20),
 WindowedSum(volume, 10) and the cross-sectional Rank of x.
 
 random_graph(n, seed): n seeded random expression trees of depth 3-8 over
@@ -9,95 +14,178 @@ compare + Select, Delay/Delta, WindowedSum/Avg/Stddev/Corr/Cov,
 TsArgMax/TsRank and DecayLinear (windows from {5, 10, 20, 30, 60}); about
 30% of the internal nodes are cross-sectional Rank or Scale."""
 import random
-from typing import List
+import types
 
-from ..Op import Builder, ConstantOp, Input, OpBase, Output, Rank, Scale
-from ..Stage import Function
-from ..ops import (Abs, Add, BackRef, DecayLinear, Div, GreaterThan, LessThan, Mul, Select, Sub, TsArgMax,
-                   TsRank, WindowedAvg, WindowedCorrelation, WindowedCovariance, WindowedStddev, WindowedSum)
-from .data import FIELDS
-
+FIELDS = ("open", "high", "low", "close", "volume", "amount")
 WINDOWS = (5, 10, 20, 30, 60)
 
+_NAMES = ["Builder", "Input", "Output", "Rank", "Scale", "ConstantOp", "Function", "Abs", "Add", "BackRef",
+          "DecayLinear", "Div", "GreaterThan", "LessThan", "Mul", "Select", "Sub", "TsArgMax", "TsArgMin",
+          "TsRank", "WindowedAvg", "WindowedCorrelation", "WindowedCovariance", "WindowedStddev", "WindowedSum",
+          "WindowedMin", "WindowedMax", "ExpMovingAvg", "WindowedQuantile", "WindowedLinearRegressionSlope",
+          "WindowedLinearRegressionRSqaure", "WindowedLinearRegressionResi", "WindowedSkew", "WindowedKurt",
+          "WindowedMaxDrawdown", "Log", "Exp", "Sqrt", "Sign", "SetInfOrNanToValue", "Max", "Min", "Pow", "Clip",
+          "And", "Or", "Not", "LessEqual", "Equals", "WindowedVar"]
 
-def simple_graph() -> Function:
-    b = Builder()
+
+def namespace(op_mod, ops_mod, stage_mod):
+    """Collect constructors from (Op, ops, Stage) modules of a front-end."""
+    ns = {}
+    for n in _NAMES:
+        for m in (op_mod, ops_mod, stage_mod):
+            if hasattr(m, n):
+                ns[n] = getattr(m, n)
+                break
+    return types.SimpleNamespace(**ns)
+
+
+def native():
+    from .. import Op, Stage, ops
+    return namespace(Op, ops, Stage)
+
+
+def simple_graph(K=None):
+    K = K or native()
+    b = K.Builder()
     with b:
-        close, opn, high, low, vol = (Input(n) for n in ("close", "open", "high", "low", "volume"))
+        close, opn, high, low, vol = (K.Input(n) for n in ("close", "open", "high", "low", "volume"))
         x = (close - opn) / ((high - low) + 0.001)
-        Output(x, "x")
-        Output(WindowedStddev(x, 20), "std20")
-        Output(WindowedSum(vol, 10), "vsum10")
-        Output(Rank(x), "rank_x")
-    return Function(b.ops)
+        K.Output(x, "x")
+        K.Output(K.WindowedStddev(x, 20), "std20")
+        K.Output(K.WindowedSum(vol, 10), "vsum10")
+        K.Output(K.Rank(x), "rank_x")
+    return K.Function(b.ops)
 
 
 class _TreeGen:
-    def __init__(self, seed: int, cs_frac: float = 0.3):
+    def __init__(self, K, seed: int, cs_frac: float = 0.3):
+        self.K = K
         self.rng = random.Random(seed)
         self.cs_frac = cs_frac
         self.inputs = {}
         self.counts = {"nodes": 0, "cs": 0, "windowed": 0}
 
-    def leaf(self) -> OpBase:
+    def leaf(self):
         name = self.rng.choice(FIELDS)
         if name not in self.inputs:
-            self.inputs[name] = Input(name)
+            self.inputs[name] = self.K.Input(name)
         return self.inputs[name]
 
     def window(self) -> int:
         return self.rng.choice(WINDOWS)
 
-    def tree(self, depth: int) -> OpBase:
+    def tree(self, depth: int):
+        K, r = self.K, self.rng
         if depth <= 0:
             return self.leaf()
-        r = self.rng
         self.counts["nodes"] += 1
         if r.random() < self.cs_frac:
             self.counts["cs"] += 1
             child = self.tree(depth - 1)
-            return Rank(child) if r.random() < 0.6 else Scale(child)
+            return K.Rank(child) if r.random() < 0.6 else K.Scale(child)
         kind = r.random()
         if kind < 0.22:
             a = self.tree(depth - 1)
             b = self.tree(r.randint(0, depth - 1))
             if r.random() < 0.5:
                 a, b = b, a
-            op = r.choice((Add, Sub, Mul, Div))
+            op = r.choice((K.Add, K.Sub, K.Mul, K.Div))
             return op(a, b)
         if kind < 0.30:
             a = self.tree(depth - 1)
             c = r.choice((0.5, 2.0, 1.5, -1.0, 10.0))
-            return r.choice((lambda x: x * c, lambda x: x + c, lambda x: Abs(x)))(a)
+            choice = r.randrange(3)
+            return a * c if choice == 0 else (a + c if choice == 1 else K.Abs(a))
         if kind < 0.36:
             a = self.tree(depth - 1)
             b = self.tree(r.randint(0, depth - 1))
-            cmp = GreaterThan(a, b) if r.random() < 0.5 else LessThan(a, b)
-            return Select(cmp, a, b)
+            cmp = K.GreaterThan(a, b) if r.random() < 0.5 else K.LessThan(a, b)
+            return K.Select(cmp, a, b)
         if kind < 0.44:
             self.counts["windowed"] += 1
             a = self.tree(depth - 1)
             w = self.window()
-            return BackRef(a, w) if r.random() < 0.5 else Sub(a, BackRef(a, w))
+            return K.BackRef(a, w) if r.random() < 0.5 else K.Sub(a, K.BackRef(a, w))
         if kind < 0.88:
             self.counts["windowed"] += 1
             a = self.tree(depth - 1)
-            op = r.choice((WindowedSum, WindowedAvg, WindowedStddev, TsArgMax, TsRank, DecayLinear))
+            op = r.choice((K.WindowedSum, K.WindowedAvg, K.WindowedStddev, K.TsArgMax, K.TsRank, K.DecayLinear))
             return op(a, self.window())
         self.counts["windowed"] += 1
         a = self.tree(depth - 1)
         b = self.tree(r.randint(0, depth - 1))
-        op = WindowedCorrelation if r.random() < 0.5 else WindowedCovariance
+        op = K.WindowedCorrelation if r.random() < 0.5 else K.WindowedCovariance
         return op(a, self.window(), b)
 
 
-def random_graph(num_factors: int, seed: int, prefix: str = "f") -> Function:
-    gen = _TreeGen(seed)
-    b = Builder()
+def random_graph(num_factors: int, seed: int, prefix: str = "f", K=None):
+    K = K or native()
+    gen = _TreeGen(K, seed)
+    b = K.Builder()
     with b:
         for i in range(num_factors):
             depth = gen.rng.randint(3, 8)
-            Output(gen.tree(depth), f"{prefix}{i:03d}")
-    f = Function(b.ops)
+            K.Output(gen.tree(depth), f"{prefix}{i:03d}")
+    f = K.Function(b.ops)
     f.workload_stats = dict(gen.counts)
     return f
+
+
+def ops_graph(K=None):
+    """One output per primitive family, over inputs a and b."""
+    K = K or native()
+    b = K.Builder()
+    with b:
+        a, c = K.Input("a"), K.Input("b")
+        K.Output(K.WindowedSum(a, 10), "sum10")
+        K.Output(K.WindowedAvg(a, 7), "avg7")
+        K.Output(K.WindowedStddev(a, 20), "std20")
+        K.Output(K.WindowedCovariance(a, 10, c), "cov10")
+        K.Output(K.WindowedCorrelation(a, 10, c), "corr10")
+        K.Output(K.WindowedMin(a, 5), "min5")
+        K.Output(K.WindowedMax(a, 5), "max5")
+        K.Output(K.TsArgMax(a, 5), "argmax5")
+        K.Output(K.TsArgMin(a, 6), "argmin6")
+        K.Output(K.TsRank(a, 7), "tsrank7")
+        K.Output(K.DecayLinear(a, 9), "decay9")
+        K.Output(K.ExpMovingAvg(a, 5), "ema5")
+        K.Output(K.WindowedQuantile(a, 10, 0.49), "q49")
+        K.Output(K.WindowedQuantile(a, 12, 0.8), "q80")
+        K.Output(K.WindowedLinearRegressionSlope(a, 10), "slope10")
+        K.Output(K.WindowedLinearRegressionRSqaure(a, 10), "rsq10")
+        K.Output(K.WindowedLinearRegressionResi(a, 10), "resi10")
+        K.Output(K.WindowedSkew(a, 5), "skew5")
+        K.Output(K.WindowedKurt(a, 6), "kurt6")
+        K.Output(K.WindowedMaxDrawdown(a, 5), "mdd5")
+        K.Output(K.BackRef(a, 3), "delay3")
+        K.Output(a - K.BackRef(a, 2), "delta2")
+        K.Output(K.Rank(a), "rank")
+        K.Output(K.Scale(c), "scale")
+        K.Output(K.Log(a), "log")
+        K.Output(K.Exp(c), "exp")
+        K.Output(K.Sqrt(a), "sqrt")
+        K.Output(K.Sign(a - c), "sign")
+        K.Output(K.SetInfOrNanToValue(a / c, 7.0), "finite")
+        K.Output(K.Select(K.GreaterThan(a, c), a, c * 2), "select")
+        K.Output(K.Max(a, c) - K.Min(a, c), "maxmin")
+        K.Output(K.Pow(a, K.ConstantOp(3)), "pow3")
+        K.Output(K.Pow(a, K.ConstantOp(0.5)), "pow05")
+        K.Output(K.Clip(a - c, 0.3), "clip")
+        K.Output(K.Select(K.And(K.Not(K.LessEqual(a, c)), K.Or(K.Equals(a, a), K.Equals(c, c))), a, c), "logic")
+    return K.Function(b.ops)
+
+
+def stream_graph(K=None):
+    """Stream-mode checks: quantile, EMA, regression slope, window stats and
+    a cross-sectional rank."""
+    K = K or native()
+    b = K.Builder()
+    with b:
+        a = K.Input("a")
+        K.Output(K.WindowedQuantile(a, 10, 0.49), "quantile")
+        K.Output(K.ExpMovingAvg(a, 10), "ema")
+        K.Output(K.WindowedLinearRegressionSlope(a, 10), "slope")
+        K.Output(K.WindowedStddev(a, 10), "std")
+        K.Output(K.TsRank(a, 5), "tsrank")
+        K.Output(K.Rank(K.WindowedSum(a, 5)), "rank")
+    return K.Function(b.ops)

@@ -1,0 +1,230 @@
+"""Writes golden vectors from the reference's own CPU path.
+
+Run in the build container only (needs /root/reference and g++):
+    bash oracle/build_ref.sh && python tests/golden/gen_ref_vectors.py
+
+For each case the reference's Python code generator (KunQuant.Driver) emits
+C++ for a graph, g++ compiles it against the reference runtime built by
+oracle/build_ref.sh (outputs stay in oracle/_ref/), and the module runs
+through the reference C API (kunRunGraph / kunStream*) on seeded inputs.
+Only inputs and outputs are committed, as .npz fixtures next to this script.
+
+These vectors pin what no closed form pins: fp32 rounding of carried state
+(Kahan rolling sums, EMA, regression sums), LogFast/Exp bit patterns, and
+end-to-end factor values of the seeded random graphs."""
+import ctypes
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF = os.environ.get("KUN_REFERENCE", "/root/reference")
+REF_OUT = os.path.join(ROOT, "oracle", "_ref")
+sys.path.insert(0, ROOT)
+
+from kunquant_amd.workloads import data as wdata  # noqa: E402
+from kunquant_amd.workloads import graphs as wgraphs  # noqa: E402
+
+
+def ref_namespace():
+    sys.path.insert(0, REF)
+    import KunQuant.Op as Op
+    import KunQuant.Stage as Stage
+    import KunQuant.ops as ops
+    return wgraphs.namespace(Op, ops, Stage)
+
+
+def ref_compile(K, f, name, cfg):
+    from KunQuant.Driver import compileit
+    srcs = compileit(f, name, **cfg)
+    gen = os.path.join(REF_OUT, "gen")
+    os.makedirs(gen, exist_ok=True)
+    cpp = os.path.join(gen, f"{name}.cpp")
+    with open(cpp, "w") as fh:
+        fh.write(srcs[0])
+    so = os.path.join(REF_OUT, f"{name}.so")
+    subprocess.check_call(["g++", "-std=c++11", "-O3", "-mavx2", "-mfma", "-fPIC", "-shared",
+                           "-fvisibility=hidden"] + CONTRACT + [ "-I", os.path.join(REF, "cpp"), cpp, "-o", so,
+                           "-L", REF_OUT, "-lKunRuntime", f"-Wl,-rpath,{REF_OUT}"])
+    return so
+
+
+# The reference's JIT passes CppCompilerConfig.other_flags to g++; the golden
+# vectors use ("-ffp-contract=off",) so that every a*b+c is rounded as two
+# IEEE operations. With GCC's default for C++ (contraction on) results depend
+# on which products GCC chooses to fuse.
+CONTRACT = ["-ffp-contract=off"]
+
+_capi = None
+
+
+def capi():
+    global _capi
+    if _capi is None:
+        L = ctypes.CDLL(os.path.join(REF_OUT, "libKunRuntime.so"), mode=ctypes.RTLD_GLOBAL)
+        vp, sz, cp = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_char_p
+        for n, res, args in [
+                ("kunCreateSingleThreadExecutor", vp, []), ("kunLoadLibrary", vp, [cp]),
+                ("kunGetModuleFromLibrary", vp, [vp, cp]), ("kunCreateBufferNameMap", vp, []),
+                ("kunSetBufferNameMap", None, [vp, cp, vp]), ("kunDestoryBufferNameMap", None, [vp]),
+                ("kunRunGraph", None, [vp, vp, vp, sz, sz, sz, sz]), ("kunCreateStream", vp, [vp, vp, sz]),
+                ("kunQueryBufferHandle", sz, [vp, cp]), ("kunStreamGetCurrentBuffer", vp, [vp, sz]),
+                ("kunStreamPushData", None, [vp, sz, vp]), ("kunStreamRun", None, [vp]),
+                ("kunDestoryStream", None, [vp])]:
+            fn = getattr(L, n)
+            fn.restype = res
+            fn.argtypes = args
+        _capi = L
+    return _capi
+
+
+def run_batch(so, name, inputs, outputs, S, T, start, L, dtype):
+    C = capi()
+    ex = C.kunCreateSingleThreadExecutor()
+    lib = C.kunLoadLibrary(so.encode())
+    mod = C.kunGetModuleFromLibrary(lib, name.encode())
+    assert mod, name
+    m = C.kunCreateBufferNameMap()
+    res = {}
+    for k, v in inputs.items():
+        C.kunSetBufferNameMap(m, k.encode(), v.ctypes.data)
+    for k in outputs:
+        res[k] = np.full((L, S), np.nan, dtype=dtype)
+        C.kunSetBufferNameMap(m, k.encode(), res[k].ctypes.data)
+    C.kunRunGraph(ex, mod, m, S, T, start, L)
+    C.kunDestoryBufferNameMap(m)
+    return res
+
+
+# The reference C API streams float data only; this harness drives the C++
+# StreamContext (RunGraph.hpp:31-53) for both element types.
+_HARNESS = r"""
+#include <Kun/RunGraph.hpp>
+#include <Kun/Module.hpp>
+using namespace kun;
+extern "C" {
+void *h_create(const void *mod, size_t S) {
+    return new StreamContext(createSingleThreadExecutor(), (const Module *)mod, S);
+}
+size_t h_query(void *c, const char *n) { return ((StreamContext *)c)->queryBufferHandle(n); }
+void h_push_f(void *c, size_t h, const float *d) { ((StreamContext *)c)->pushData(h, d); }
+void h_push_d(void *c, size_t h, const double *d) { ((StreamContext *)c)->pushData(h, d); }
+void h_run(void *c) { ((StreamContext *)c)->run(); }
+const float *h_get_f(void *c, size_t h) { return ((StreamContext *)c)->getCurrentBufferPtrFloat(h); }
+const double *h_get_d(void *c, size_t h) { return ((StreamContext *)c)->getCurrentBufferPtrDouble(h); }
+void h_free(void *c) { delete (StreamContext *)c; }
+}
+"""
+
+
+def harness():
+    src = os.path.join(REF_OUT, "gen", "stream_harness.cpp")
+    so = os.path.join(REF_OUT, "stream_harness.so")
+    os.makedirs(os.path.dirname(src), exist_ok=True)
+    with open(src, "w") as fh:
+        fh.write(_HARNESS)
+    subprocess.check_call(["g++", "-std=c++11", "-O2", "-mavx2", "-mfma", "-ffp-contract=off", "-fPIC", "-shared", "-I",
+                           os.path.join(REF, "cpp"), src, "-o", so, "-L", REF_OUT, "-lKunRuntime",
+                           f"-Wl,-rpath,{REF_OUT}"])
+    H = ctypes.CDLL(so)
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    for n, res, args in [("h_create", vp, [vp, sz]), ("h_query", sz, [vp, ctypes.c_char_p]),
+                         ("h_push_f", None, [vp, sz, vp]), ("h_push_d", None, [vp, sz, vp]), ("h_run", None, [vp]),
+                         ("h_get_f", vp, [vp, sz]), ("h_get_d", vp, [vp, sz]), ("h_free", None, [vp])]:
+        fn = getattr(H, n)
+        fn.restype = res
+        fn.argtypes = args
+    return H
+
+
+def run_stream(so, name, rows, outputs, S, dtype):
+    C = capi()
+    H = harness()
+    lib = C.kunLoadLibrary(so.encode())
+    mod = C.kunGetModuleFromLibrary(lib, name.encode())
+    st = H.h_create(mod, S)
+    ha = H.h_query(st, b"a")
+    hs = {k: H.h_query(st, k.encode()) for k in outputs}
+    dbl = dtype == np.float64
+    ct = ctypes.c_double if dbl else ctypes.c_float
+    res = {k: np.empty((len(rows), S), dtype=dtype) for k in outputs}
+    for i, r in enumerate(rows):
+        r = np.ascontiguousarray(r, dtype=dtype)
+        (H.h_push_d if dbl else H.h_push_f)(st, ha, r.ctypes.data)
+        H.h_run(st)
+        for k in outputs:
+            p = (H.h_get_d if dbl else H.h_get_f)(st, hs[k])
+            res[k][i] = np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ct)), shape=(S,))
+    H.h_free(st)
+    return res
+
+
+def output_names(f):
+    return [op.attrs["name"] for op in f.ops if op.__class__.__name__ == "Output"]
+
+
+def ops_inputs(S, T, dtype, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    a = (rng.random((T, S)) + 0.5).astype(dtype)
+    b = (rng.random((T, S)) - 0.3).astype(dtype)
+    a[7, 3] = np.nan
+    b[11, 5] = np.nan
+    a[20:23, :] = a[19, :]  # ties in windows
+    return {"a": a, "b": b}
+
+
+CASES = [
+    # name, graph factory, compile config, dtype, inputs(S, T), S, T, start, L
+    ("simple_f32", wgraphs.simple_graph, dict(input_layout="TS", output_layout="TS"), np.float32,
+     lambda: {k: v for k, v in wdata.ohlcv(64, 260, seed=1).items() if k != "amount"}, 64, 260, 0, 260),
+    ("ops_f32", wgraphs.ops_graph, dict(input_layout="TS", output_layout="TS"), np.float32,
+     lambda: ops_inputs(40, 120, np.float32, 7), 40, 120, 10, 110),
+    ("ops_f64", wgraphs.ops_graph, dict(dtype="double", input_layout="TS", output_layout="TS",
+                                        options={"no_fast_stat": "no_warn"}), np.float64,
+     lambda: ops_inputs(40, 120, np.float64, 8), 40, 120, 10, 110),
+    ("rand_f32", lambda K: wgraphs.random_graph(100, 102, K=K), dict(input_layout="TS", output_layout="TS"),
+     np.float32, lambda: wdata.ohlcv(24, 130, seed=2, nan_frac=0.1), 24, 130, 0, 130),
+    ("rand_f64", lambda K: wgraphs.random_graph(40, 103, K=K),
+     dict(dtype="double", input_layout="TS", output_layout="TS", options={"no_fast_stat": "no_warn"}),
+     np.float64, lambda: wdata.ohlcv(24, 130, seed=3, dtype=np.float64, nan_frac=0.1), 24, 130, 0, 130),
+]
+
+STREAM_CASES = [
+    ("stream_f64", dict(dtype="double", input_layout="STREAM", output_layout="STREAM"), np.float64),
+    ("stream_f32", dict(input_layout="STREAM", output_layout="STREAM", options={"opt_reduce": False}), np.float32),
+]
+
+
+def main():
+    if not os.path.exists(os.path.join(REF_OUT, "libKunRuntime.so")):
+        subprocess.check_call(["bash", os.path.join(ROOT, "oracle", "build_ref.sh")])
+    K = ref_namespace()
+    for name, factory, cfg, dtype, make_inputs, S, T, start, L in CASES:
+        f = factory(K)
+        outs = output_names(f)
+        so = ref_compile(K, f, name, dict(cfg))
+        inputs = make_inputs()
+        res = run_batch(so, name, inputs, outs, S, T, start, L, dtype)
+        np.savez_compressed(os.path.join(HERE, f"ref_{name}.npz"),
+                            **{f"in_{k}": v for k, v in inputs.items()}, **{f"out_{k}": v for k, v in res.items()},
+                            meta=np.array([S, T, start, L]))
+        print("wrote", name, len(outs), "outputs")
+    rng = np.random.Generator(np.random.PCG64(11))
+    for name, cfg, dtype in STREAM_CASES:
+        f = wgraphs.stream_graph(K)
+        outs = output_names(f)
+        so = ref_compile(K, f, name, dict(cfg))
+        S = 16
+        rows = rng.random((60, S)).astype(dtype)
+        rows[13, 2] = np.nan
+        res = run_stream(so, name, rows, outs, S, dtype)
+        np.savez_compressed(os.path.join(HERE, f"ref_{name}.npz"), in_a=rows,
+                            **{f"out_{k}": v for k, v in res.items()}, meta=np.array([S, len(rows)]))
+        print("wrote", name)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,62 @@
+"""The oracle against golden vectors written by the reference's own CPU path
+(tests/golden/gen_ref_vectors.py: KunQuant's code generator + runtime built
+from its sources). Values must match exactly; NaN positions must match
+(NaN payload bits are not compared)."""
+import os
+
+import numpy as np
+import pytest
+
+from kunquant_amd.workloads import catalog, graphs
+from kunquant_amd.Driver import compileit
+from kunquant_amd.codegen.program import program_text
+from oracle.kun_oracle import OracleProgram, OracleStream
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+BATCH = {
+    "simple_f32": (graphs.simple_graph, dict(input_layout="TS", output_layout="TS")),
+    "ops_f32": (graphs.ops_graph, dict(input_layout="TS", output_layout="TS")),
+    "ops_f64": (graphs.ops_graph, dict(dtype="double", input_layout="TS", output_layout="TS",
+                                       options={"no_fast_stat": "no_warn"})),
+    "rand_f32": (lambda: graphs.random_graph(100, 102), dict(input_layout="TS", output_layout="TS")),
+    "rand_f64": (lambda: graphs.random_graph(40, 103), dict(dtype="double", input_layout="TS", output_layout="TS",
+                                                            options={"no_fast_stat": "no_warn"})),
+}
+STREAM = {
+    "stream_f64": dict(dtype="double", input_layout="STREAM", output_layout="STREAM"),
+    "stream_f32": dict(input_layout="STREAM", output_layout="STREAM", options={"opt_reduce": False}),
+}
+
+
+def load(name):
+    z = np.load(os.path.join(GOLDEN, f"ref_{name}.npz"))
+    ins = {k[3:]: z[k] for k in z.files if k.startswith("in_")}
+    outs = {k[4:]: z[k] for k in z.files if k.startswith("out_")}
+    return ins, outs, z["meta"]
+
+
+def same_values(a, b):
+    return np.array_equal(a, b, equal_nan=True)
+
+
+@pytest.mark.parametrize("name", sorted(BATCH))
+def test_oracle_matches_reference_batch(name):
+    factory, cfg = BATCH[name]
+    ins, outs, (S, T, start, L) = load(name)
+    prog = OracleProgram(program_text(compileit(factory(), name, **cfg)))
+    got = prog.run(ins, int(S), int(T), int(start), int(L), nthreads=4)
+    bad = [k for k in outs if not same_values(got[k], outs[k])]
+    assert not bad, f"{len(bad)} outputs differ from the reference: {bad[:8]}"
+
+
+@pytest.mark.parametrize("name", sorted(STREAM))
+def test_oracle_matches_reference_stream(name):
+    ins, outs, (S, n) = load(name)
+    prog = OracleProgram(program_text(compileit(graphs.stream_graph(), name, **STREAM[name])))
+    st = OracleStream(prog, int(S))
+    for i in range(int(n)):
+        st.push("a", ins["a"][i])
+        st.run()
+        for k in outs:
+            assert same_values(st.get(k), outs[k][i]), (k, i)

@@ -139,6 +139,20 @@ KUN_API int kunModuleBufferWindow(KunModuleHandle m, int idx);
 KUN_API int kunModuleNumStages(KunModuleHandle m);
 /* KUNPROG text describing the module's stage IR */
 KUN_API const char *kunModuleProgram(KunModuleHandle m);
+/* stage table: kind 0 time-series, 1 Rank, 2 Scale; inputs/outputs are buffer indices */
+KUN_API const char *kunModuleStageName(KunModuleHandle m, int stage);
+KUN_API int kunModuleStageKind(KunModuleHandle m, int stage);
+KUN_API int kunModuleStageNumInputs(KunModuleHandle m, int stage);
+KUN_API int kunModuleStageInput(KunModuleHandle m, int stage, int j);
+KUN_API int kunModuleStageNumOutputs(KunModuleHandle m, int stage);
+KUN_API int kunModuleStageOutput(KunModuleHandle m, int stage, int j);
+/* Per-stage kernel timing with HIP events recorded on the stream each stage
+ * kernel is launched on. While enabled, every batch run records an event
+ * pair around each stage; kunExecutorStageTime synchronises and returns the
+ * accumulated milliseconds and launch count of one stage. */
+KUN_API int kunExecutorEnableProfiling(KunExecutorHandle exec, int on);
+KUN_API int kunExecutorStageTime(KunExecutorHandle exec, KunModuleHandle m, int stage, double *total_ms,
+                                 long *count);
 /* device pointer of the newest row of a stream buffer (num_stocks elements) */
 KUN_API const void *kunStreamGetCurrentDeviceBuffer(KunStreamContextHandle context, size_t handle);
 /* number of rows consumed by the stream so far */

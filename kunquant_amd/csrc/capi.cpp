@@ -115,6 +115,35 @@ KUN_API int kunModuleBufferWindow(KunModuleHandle m, int idx) { return unwrap_mo
 KUN_API int kunModuleNumStages(KunModuleHandle m) { return unwrap_mod(m)->nstage; }
 KUN_API const char *kunModuleProgram(KunModuleHandle m) { return unwrap_mod(m)->program; }
 
+KUN_API const char *kunModuleStageName(KunModuleHandle m, int i) { return unwrap_mod(m)->stages[i].name; }
+KUN_API int kunModuleStageKind(KunModuleHandle m, int i) { return unwrap_mod(m)->stages[i].kind; }
+KUN_API int kunModuleStageNumInputs(KunModuleHandle m, int i) { return unwrap_mod(m)->stages[i].nin; }
+KUN_API int kunModuleStageInput(KunModuleHandle m, int i, int j) { return unwrap_mod(m)->stages[i].in[j]; }
+KUN_API int kunModuleStageNumOutputs(KunModuleHandle m, int i) { return unwrap_mod(m)->stages[i].nout; }
+KUN_API int kunModuleStageOutput(KunModuleHandle m, int i, int j) { return unwrap_mod(m)->stages[i].out[j]; }
+
+KUN_API int kunExecutorEnableProfiling(KunExecutorHandle exec, int on) {
+    return guarded([&] {
+        Executor &ex = **unwrap_exec(exec);
+        std::lock_guard<std::mutex> g(ex.mu);
+        ex.collect_profile();
+        ex.profiling = on != 0;
+        if (!on) ex.stage_ms.clear();
+    });
+}
+
+KUN_API int kunExecutorStageTime(KunExecutorHandle exec, KunModuleHandle m, int stage, double *total_ms,
+                                 long *count) {
+    return guarded([&] {
+        Executor &ex = **unwrap_exec(exec);
+        std::lock_guard<std::mutex> g(ex.mu);
+        ex.collect_profile();
+        auto it = ex.stage_ms.find({(const void *)m, stage});
+        *total_ms = it == ex.stage_ms.end() ? 0.0 : it->second.first;
+        *count = it == ex.stage_ms.end() ? 0 : it->second.second;
+    });
+}
+
 // ---- stream contexts ----------------------------------------------------
 KUN_API KunStatus kunCreateStreamEx(KunExecutorHandle exec, KunModuleHandle m, size_t num_stocks,
                                     const KunStreamExtraArgs *extra_args, KunStreamContextHandle *out_handle) {

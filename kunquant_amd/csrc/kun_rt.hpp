@@ -47,6 +47,18 @@ struct Executor {
     std::mutex mu;
     // temporaries of device-resident runs, per (module, stocks, rows)
     std::map<std::tuple<const void *, size_t, size_t>, std::vector<void *>> temps;
+    // per-stage kernel timing (bench / diagnostics)
+    bool profiling = false;
+    struct Pending {
+        const void *module;
+        int stage;
+        hipEvent_t start, stop;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::map<std::pair<const void *, int>, std::pair<double, long>> stage_ms;
+    hipEvent_t take_event();
+    void collect_profile();
     Executor();
     ~Executor();
     void free_temps();

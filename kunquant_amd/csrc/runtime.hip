@@ -76,7 +76,38 @@ Executor::Executor() {
 
 Executor::~Executor() {
     hipStreamSynchronize(stream);
+    for (auto &p : pending) {
+        hipEventDestroy(p.start);
+        hipEventDestroy(p.stop);
+    }
+    for (auto e : event_pool) hipEventDestroy(e);
+    free_temps();
     hipStreamDestroy(stream);
+}
+
+hipEvent_t Executor::take_event() {
+    if (!event_pool.empty()) {
+        hipEvent_t e = event_pool.back();
+        event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    HIPCHECK(hipEventCreate(&e));
+    return e;
+}
+
+void Executor::collect_profile() {
+    for (auto &p : pending) {
+        HIPCHECK(hipEventSynchronize(p.stop));
+        float ms = 0;
+        HIPCHECK(hipEventElapsedTime(&ms, p.start, p.stop));
+        auto &acc = stage_ms[{p.module, p.stage}];
+        acc.first += ms;
+        acc.second += 1;
+        event_pool.push_back(p.start);
+        event_pool.push_back(p.stop);
+    }
+    pending.clear();
 }
 
 // ---------------------------------------------------------------------------
@@ -166,10 +197,16 @@ static kun_cs::View cs_view(const KunModuleDesc *m, int b, void *ptr, size_t T, 
 // Enqueue every stage of a batch module; dev[b] are device pointers,
 // scratch[s] the HBM scratch of stage s (or null).
 void enqueue_batch(const KunModuleDesc *m, const std::vector<void *> &dev, const std::vector<void *> &scratch,
-                   size_t S, size_t T, size_t start, size_t L, hipStream_t st) {
+                   size_t S, size_t T, size_t start, size_t L, hipStream_t st, Executor *prof) {
     if (L == 0) return;
     for (int si : stage_order(m)) {
         const KunStageDesc &sd = m->stages[si];
+        Executor::Pending ev{m, si, nullptr, nullptr};
+        if (prof) {
+            ev.start = prof->take_event();
+            ev.stop = prof->take_event();
+            HIPCHECK(hipEventRecord(ev.start, st));
+        }
         if (sd.kind == KUN_STAGE_TS) {
             kun::StageArgs a;
             memset(&a, 0, sizeof a);
@@ -196,6 +233,10 @@ void enqueue_batch(const KunModuleDesc *m, const std::vector<void *> &dev, const
                                ? kun_cs::launch_rank(m->dtype, vi, vo, (long)S, (long)L, m->blocking, st)
                                : kun_cs::launch_scale(m->dtype, vi, vo, (long)S, (long)L, m->blocking, st);
             if (e != hipSuccess) throw Error(std::string("cross-sectional kernel: ") + hipGetErrorString(e));
+        }
+        if (prof) {
+            HIPCHECK(hipEventRecord(ev.stop, st));
+            prof->pending.push_back(ev);
         }
     }
 }
@@ -250,7 +291,7 @@ void run_graph_host(Executor &ex, const KunModuleDesc *m, const NameMap &bufs, s
     for (int i = 0; i < m->nstage; i++)
         if (m->stages[i].kind == KUN_STAGE_TS && m->stages[i].state_bytes)
             scratch[i] = ex.pool.get((size_t)m->stages[i].state_bytes * S);
-    enqueue_batch(m, dev, scratch, S, T, start, L, ex.stream);
+    enqueue_batch(m, dev, scratch, S, T, start, L, ex.stream, ex.profiling ? &ex : nullptr);
     for (int b = 0; b < m->nbuf; b++) {
         const KunBufferDesc &d = m->bufs[b];
         if (d.kind == KUN_BUF_OUTPUT)
@@ -293,7 +334,7 @@ void run_graph_device(Executor &ex, const KunModuleDesc *m, const NameMap &bufs,
             dev[b] = it->second;
         }
     }
-    enqueue_batch(m, dev, scratch, S, T, start, L, st);
+    enqueue_batch(m, dev, scratch, S, T, start, L, st, ex.profiling ? &ex : nullptr);
 }
 
 void Executor::free_temps() {

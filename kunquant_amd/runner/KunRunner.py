@@ -55,6 +55,13 @@ def _load():
     for n in ("kunModuleBufferKind", "kunModuleBufferUnreliable", "kunModuleBufferWindow"):
         sig(n, i, vp, i)
     sig("kunModuleProgram", cp, vp)
+    sig("kunModuleStageName", cp, vp, i)
+    for n in ("kunModuleStageKind", "kunModuleStageNumInputs", "kunModuleStageNumOutputs"):
+        sig(n, i, vp, i)
+    sig("kunModuleStageInput", i, vp, i, i)
+    sig("kunModuleStageOutput", i, vp, i, i)
+    sig("kunExecutorEnableProfiling", i, vp, i)
+    sig("kunExecutorStageTime", i, vp, vp, i, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long))
     sig("kunCreateStreamEx", i, vp, vp, sz, vp, ctypes.POINTER(vp))
     sig("kunStreamSerializeStates", i, vp, i, vp, ctypes.POINTER(sz))
     sig("kunQueryBufferHandle", sz, vp, cp)
@@ -90,6 +97,18 @@ class Executor:
     def synchronize(self):
         if _load().kunExecutorSynchronize(self._h):
             raise RuntimeError(_err())
+
+    def enable_profiling(self, on: bool = True):
+        """Record HIP events around every stage kernel of later batch runs."""
+        if _load().kunExecutorEnableProfiling(self._h, 1 if on else 0):
+            raise RuntimeError(_err())
+
+    def stage_time(self, mod: "Module", stage: int):
+        """(accumulated milliseconds, launches) of one stage kernel."""
+        ms, n = ctypes.c_double(), ctypes.c_long()
+        if _load().kunExecutorStageTime(self._h, mod._h, stage, ctypes.byref(ms), ctypes.byref(n)):
+            raise RuntimeError(_err())
+        return ms.value, n.value
 
 
 def createSingleThreadExecutor() -> Executor:
@@ -140,6 +159,17 @@ class Module:
 
     def program(self) -> str:
         return _load().kunModuleProgram(self._h).decode()
+
+    def stages(self):
+        """[(name, kind, input buffer indices, output buffer indices)]; kind
+        0 time-series, 1 Rank, 2 Scale."""
+        L = _load()
+        res = []
+        for i in range(L.kunModuleNumStages(self._h)):
+            ins = [L.kunModuleStageInput(self._h, i, j) for j in range(L.kunModuleStageNumInputs(self._h, i))]
+            outs = [L.kunModuleStageOutput(self._h, i, j) for j in range(L.kunModuleStageNumOutputs(self._h, i))]
+            res.append((L.kunModuleStageName(self._h, i).decode(), L.kunModuleStageKind(self._h, i), ins, outs))
+        return res
 
 
 class Library:

@@ -6,7 +6,7 @@ import os
 from typing import Callable, Dict, Tuple
 
 from .. import PACKAGE_DIR
-from .graphs import ops_graph, random_graph, simple_graph, stream_graph
+from .graphs import aligned_graph, ops_graph, random_graph, simple_graph, stream_graph
 
 MODULE_DIR = os.path.join(PACKAGE_DIR, "lib", "modules")
 
@@ -20,6 +20,11 @@ CATALOG: Dict[str, Tuple[Callable, dict]] = {
     "cfg2_f32": (lambda: random_graph(100, 102), dict(input_layout="TS", output_layout="TS")),
     "cfg2_f64": (lambda: random_graph(100, 102), dict(dtype="double", input_layout="TS", output_layout="TS",
                                                         options={"no_fast_stat": "no_warn"})),
+    "rand40_f64": (lambda: random_graph(40, 103), dict(dtype="double", input_layout="TS", output_layout="TS",
+                                                        options={"no_fast_stat": "no_warn"})),
+    "streamgraph_batch_f64": (stream_graph, dict(dtype="double", input_layout="TS", output_layout="TS",
+                                                  options={"no_fast_stat": True})),
+    "aligned_f32": (aligned_graph, dict(input_layout="TS", output_layout="TS", allow_unaligned=False)),
     "stream_f64": (stream_graph, dict(dtype="double", input_layout="STREAM", output_layout="STREAM")),
     "stream_f32": (stream_graph, dict(input_layout="STREAM", output_layout="STREAM",
                                       options={"opt_reduce": False})),

@@ -189,3 +189,12 @@ def stream_graph(K=None):
         K.Output(K.TsRank(a, 5), "tsrank")
         K.Output(K.Rank(K.WindowedSum(a, 5)), "rank")
     return K.Function(b.ops)
+
+
+def aligned_graph(K=None):
+    """Rank(a) * 2; compiled with allow_unaligned=False."""
+    K = K or native()
+    b = K.Builder()
+    with b:
+        K.Output(K.Rank(K.Input("a")) * 2, "ou2")
+    return K.Function(b.ops)

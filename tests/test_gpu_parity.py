@@ -1,0 +1,125 @@
+"""Device results against the reference's golden vectors and the oracle.
+
+Bar: every output value equal (bit-exact for ranks, arg-indices, masks and
+all floating-point values; NaN positions equal, NaN payload bits ignored).
+Runs through the C ABI (libkunhip.so) via KunRunner."""
+import os
+
+import numpy as np
+import pytest
+
+from kunquant_amd.codegen.program import program_text
+from kunquant_amd.runner import KunRunner as kr
+from kunquant_amd.workloads import catalog, data
+from oracle.kun_oracle import OracleProgram
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+NTHREADS = 16
+
+
+@pytest.fixture(scope="module")
+def executor():
+    return kr.createSingleThreadExecutor()
+
+
+def golden(name):
+    z = np.load(os.path.join(GOLDEN, f"ref_{name}.npz"))
+    return ({k[3:]: z[k] for k in z.files if k.startswith("in_")},
+            {k[4:]: z[k] for k in z.files if k.startswith("out_")}, [int(x) for x in z["meta"]])
+
+
+def assert_same(got, exp, what):
+    bad = [k for k in exp if not np.array_equal(got[k], exp[k], equal_nan=True)]
+    if bad:
+        k = bad[0]
+        a, b = got[k], exp[k]
+        fin = ~np.isnan(a) & ~np.isnan(b)
+        msg = (f"{what}: {len(bad)}/{len(exp)} outputs differ (first: {k}, NaN masks equal: "
+               f"{np.array_equal(np.isnan(a), np.isnan(b))}, values differing: {int((a[fin] != b[fin]).sum())})")
+        raise AssertionError(msg)
+
+
+@pytest.mark.parametrize("golden_name,module", [("simple_f32", "simple_f32"), ("ops_f32", "ops_f32"),
+                                                ("ops_f64", "ops_f64"), ("rand_f32", "cfg2_f32"),
+                                                ("rand_f64", "rand40_f64")])
+def test_device_matches_reference_golden(executor, golden_name, module):
+    ins, outs, (S, T, start, L) = golden(golden_name)
+    _, mod = catalog.load(module)
+    got = kr.runGraph(executor, mod, {k: ins[k] for k in mod.getInputNames()}, start, L)
+    assert_same(got, outs, f"{module} vs reference")
+
+
+def _oracle(module):
+    return OracleProgram(program_text(catalog.module_ir(module)))
+
+
+@pytest.mark.parametrize("module,S,T,start,L,nan_frac", [
+    ("cfg2_f32", 1000, 300, 17, 250, 0.1),   # unaligned stocks, cur_time > 0, NaN cells
+    ("cfg2_f64", 333, 200, 0, 200, 0.1),     # fp64 fast statistics
+    ("simple_f32", 8, 40, 0, 40, 0.0),       # fewer stocks than a wavefront
+    ("simple_f32", 513, 70, 5, 60, 0.02),
+])
+def test_device_matches_oracle(executor, module, S, T, start, L, nan_frac):
+    _, mod = catalog.load(module)
+    d = data.ohlcv(S, T, seed=S + T, dtype=mod.dtype, nan_frac=nan_frac)
+    ins = {k: d[k] for k in mod.getInputNames()}
+    got = kr.runGraph(executor, mod, ins, start, L)
+    exp = _oracle(module).run(ins, S, T, start, L, nthreads=NTHREADS)
+    assert_same(got, exp, f"{module} {S}x{T}")
+
+
+@pytest.mark.parametrize("module", ["ops_f32", "ops_f64"])
+def test_ops_edge_values(executor, module):
+    """Ties, NaN cells, zeros and infinities through every primitive."""
+    _, mod = catalog.load(module)
+    S, T = 70, 90
+    rng = np.random.default_rng(21)
+    a = np.round(rng.random((T, S)) * 4, 1).astype(mod.dtype) + 0.5
+    b = (rng.random((T, S)) - 0.5).astype(mod.dtype)
+    a[rng.random((T, S)) < 0.03] = np.nan
+    b[rng.random((T, S)) < 0.03] = np.nan
+    b[5, :] = 0.0
+    a[9, :3] = np.inf
+    b[12, 4:8] = -0.0
+    ins = {"a": a, "b": b}
+    got = kr.runGraph(executor, mod, ins, 0, T)
+    exp = _oracle(module).run(ins, S, T, 0, T, nthreads=NTHREADS)
+    assert_same(got, exp, module)
+
+
+def test_sts_layout(executor):
+    """STs input/output layout ([S/8, T, 8]); padding lanes are not compared."""
+    _, mod = catalog.load("ops_f32_sts")
+    S, T, b = 100, 80, mod.blocking_len
+    Sp = (S + b - 1) // b * b
+    rng = np.random.default_rng(22)
+    ts = {"a": rng.random((T, S)).astype(np.float32) + 0.5, "b": rng.random((T, S)).astype(np.float32) - 0.3}
+
+    def to_sts(x):
+        xp = np.zeros((T, Sp), dtype=x.dtype)
+        xp[:, :S] = x
+        return np.ascontiguousarray(xp.reshape(T, Sp // b, b).transpose(1, 0, 2))
+
+    ins = {k: to_sts(v) for k, v in ts.items()}
+    got = kr.runGraph(executor, mod, ins, 0, T, num_stocks=S)
+    exp = _oracle("ops_f32_sts").run(ins, S, T, 0, T, nthreads=NTHREADS)
+    unb = lambda x: x.transpose(1, 0, 2).reshape(T, Sp)[:, :S]
+    assert_same({k: unb(v) for k, v in got.items()}, {k: unb(v) for k, v in exp.items()}, "STs")
+
+
+def test_full_size_config2(executor):
+    """8192 stocks x 1024 rows: the first 128 rows equal the oracle's (rows
+    only depend on earlier rows), a second run is bit-identical, and every
+    Rank output row averages to (n+1)/(2n)."""
+    _, mod = catalog.load("cfg2_f32")
+    S, T, P = 8192, 1024, 128
+    d = data.ohlcv(S, T, seed=2, nan_frac=0.1)
+    got = kr.runGraph(executor, mod, d, 0, T)
+    again = kr.runGraph(executor, mod, d, 0, T)
+    for k in got:
+        assert np.array_equal(got[k].view(np.uint32), again[k].view(np.uint32)), k
+    prefix = {k: np.ascontiguousarray(v[:P]) for k, v in d.items()}
+    exp = _oracle("cfg2_f32").run(prefix, S, P, 0, P, nthreads=NTHREADS)
+    assert_same({k: v[:P] for k, v in got.items()}, exp, "config-2 prefix")

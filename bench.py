@@ -68,32 +68,46 @@ def main():
     cells = len(outputs) * S * T
     value = cells / (ms / 1e3)
 
-    # per-stage kernel time (HIP events on the executor stream the kernels run on)
+    # per-kernel time (HIP events on the executor stream the kernels run on)
     ex.enable_profiling(True)
     prof_steps = max(1, min(3, args.steps))
     for _ in range(prof_steps):
         kr.runGraphDevice(ex, mod, ptrs, S, T, 0, T)
     ex.synchronize()
     stages = mod.stages()
-    per_stage = []
-    for i, (name, kind, ins, outs_) in enumerate(stages):
-        tot, n = ex.stage_time(mod, i)
-        per_stage.append((tot / max(n, 1), i, kind, ins, outs_))
-    ex.enable_profiling(False)
-    per_stage.sort(reverse=True)
-    stage_sum = sum(p[0] for p in per_stage)
-    dom_ms, dom_i, dom_kind, dom_in, dom_out = per_stage[0]
-    # algorithmic bytes of the dominant kernel: each input row read once, each output row written once
     bufs = mod.buffers
+
     def buf_bytes(b):
-        name = bufs[b][0]
-        return S * esz if name.startswith("__init") else S * T * esz
-    dom_bytes = sum(buf_bytes(b) for b in dom_in) + sum(buf_bytes(b) for b in dom_out)
+        return S * esz if bufs[b][0].startswith("__init") else S * T * esz
+
+    per_kernel = []  # (avg ms, stage, kernel, kind, algorithmic bytes)
+    stage_ms = {}
+    for i, (name, kind, ins, outs_) in enumerate(stages):
+        kinfo = mod.kernels(i)
+        for j in range(max(1, len(kinfo))):
+            tot, n = ex.kernel_time(mod, i, j)
+            avg = tot / max(n, 1)
+            if kind == 0:
+                # each distinct HBM value the kernel reads or writes, once per (stock, row)
+                nbytes = kinfo[j][4] * S * T * esz
+                label = ("chunked" if kinfo[j][0] == 0 else "sequential") + " time-series"
+            else:
+                # cross-sectional: the input row read once, the output row written once
+                nbytes = sum(buf_bytes(b) for b in ins) + sum(buf_bytes(b) for b in outs_)
+                label = {1: "rank", 2: "scale"}[kind]
+            per_kernel.append((avg, i, j, label, nbytes))
+            stage_ms[i] = stage_ms.get(i, 0.0) + avg
+    ex.enable_profiling(False)
+    per_kernel.sort(reverse=True)
+    kernel_sum = sum(p[0] for p in per_kernel)
+    dom_ms, dom_i, dom_j, dom_label, dom_bytes = per_kernel[0]
     achieved = dom_bytes / (dom_ms / 1e3) / 1e9
     graph_bytes = (len(inputs) + len(outputs)) * S * T * esz
     graph_gbs = graph_bytes / (ms / 1e3) / 1e9
+    by_kind = {}
+    for p in per_kernel:
+        by_kind[p[3]] = by_kind.get(p[3], 0.0) + p[0]
 
-    kinds = {0: "time-series", 1: "rank", 2: "scale"}
     result = {
         "metric": "factor-cells/s",
         "value": value,
@@ -113,11 +127,13 @@ def main():
                    "stages": len(stages)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": f"stage {dom_i} ({kinds[dom_kind]})", "kernel_ms": dom_ms,
+                     "kernel": f"stage {dom_i} kernel {dom_j} ({dom_label})", "kernel_ms": dom_ms,
                      "algorithmic_bytes": dom_bytes},
         "graph": {"algorithmic_bytes": graph_bytes, "achieved_GBs": graph_gbs,
-                  "frac_of_hbm_peak": graph_gbs / HBM_PEAK_GBS, "sum_stage_kernel_ms": stage_sum,
-                  "top_stages_ms": [[p[1], kinds[p[2]], round(p[0], 4)] for p in per_stage[:5]]},
+                  "frac_of_hbm_peak": graph_gbs / HBM_PEAK_GBS, "sum_kernel_ms": kernel_sum,
+                  "kernel_ms_by_kind": {k: round(v, 3) for k, v in by_kind.items()},
+                  "launches_per_step": len(per_kernel),
+                  "top_kernels_ms": [[p[1], p[2], p[3], round(p[0], 4)] for p in per_kernel[:8]]},
     }
 
     if not args.no_cpu_baseline:

@@ -121,6 +121,12 @@ KUN_API int kunRunGraphEx(KunExecutorHandle exec, KunModuleHandle m, KunBufferNa
 KUN_API int kunRunGraphDevice(KunExecutorHandle exec, KunModuleHandle m, KunBufferNameMapHandle buffers,
                               size_t num_stocks, size_t total_time, size_t cur_time, size_t length,
                               void *hip_stream);
+/* How row-chunked kernels split the rows of a batch run: about
+ * `target_workgroups` workgroups per kernel, each chunk at least
+ * `halo_factor` times the history it recomputes and `min_rows` rows long.
+ * Results do not depend on it. Drops cached device temporaries. */
+KUN_API int kunExecutorSetChunking(KunExecutorHandle exec, long target_workgroups, long halo_factor,
+                                  long min_rows);
 /* Blocks until the executor's stream is idle. */
 KUN_API int kunExecutorSynchronize(KunExecutorHandle exec);
 
@@ -146,13 +152,24 @@ KUN_API int kunModuleStageNumInputs(KunModuleHandle m, int stage);
 KUN_API int kunModuleStageInput(KunModuleHandle m, int stage, int j);
 KUN_API int kunModuleStageNumOutputs(KunModuleHandle m, int stage);
 KUN_API int kunModuleStageOutput(KunModuleHandle m, int stage, int j);
-/* Per-stage kernel timing with HIP events recorded on the stream each stage
- * kernel is launched on. While enabled, every batch run records an event
- * pair around each stage; kunExecutorStageTime synchronises and returns the
- * accumulated milliseconds and launch count of one stage. */
+/* kernels a stage launches per batch run (cross-sectional stages: 1), and
+ * the plan of one kernel of a time-series stage: kind 0 row-chunked, 1
+ * sequential over rows; ncomp independent op groups; halo rows recomputed
+ * before each chunk; static LDS bytes per workgroup; distinct HBM values
+ * the kernel reads or writes per (stock, row). */
+KUN_API int kunModuleStageNumKernels(KunModuleHandle m, int stage);
+KUN_API int kunModuleKernelInfo(KunModuleHandle m, int stage, int kern, int *kind, int *ncomp, int *halo,
+                                int *lds_bytes, int *io_values);
+/* Kernel timing with HIP events recorded on the stream each kernel is
+ * launched on. While enabled, every batch run records an event pair around
+ * each kernel; kunExecutorStageTime synchronises and returns the accumulated
+ * milliseconds of all kernels of one stage and the number of runs;
+ * kunExecutorKernelTime the same for one kernel of the stage. */
 KUN_API int kunExecutorEnableProfiling(KunExecutorHandle exec, int on);
 KUN_API int kunExecutorStageTime(KunExecutorHandle exec, KunModuleHandle m, int stage, double *total_ms,
                                  long *count);
+KUN_API int kunExecutorKernelTime(KunExecutorHandle exec, KunModuleHandle m, int stage, int kern, double *total_ms,
+                                  long *count);
 /* device pointer of the newest row of a stream buffer (num_stocks elements) */
 KUN_API const void *kunStreamGetCurrentDeviceBuffer(KunStreamContextHandle context, size_t handle);
 /* number of rows consumed by the stream so far */

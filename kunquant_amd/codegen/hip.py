@@ -1,31 +1,38 @@
 """HIP emitter: lowers the stages of a ModuleIR to gfx950 kernels built from
 the device templates in csrc/kun_device.hpp.
 
-Batch time-series stage kernel (one workgroup = 64 stocks, one lane per
-stock, TS rows coalesced):
-  * every value read through a window (IterValue, BackRef, FastWindowedSum,
-    WindowedLinearRegression sources) keeps a per-lane history ring in LDS
-    laid out [slot][lane] (bank-conflict free), slot count a power of two,
-    NaN-initialised -- exactly the reference's "no history before start"
-    rule (Ops.hpp:87-92, 182-186);
-  * carried state (FastWindowedSum, EMA, accumulators, regression sums,
-    sorted windows) lives in registers / LDS for the whole time loop;
+Batch time-series stages follow the kernel plan of codegen/plan.py: each
+level becomes one kernel; its independent components are selected by
+blockIdx.z, blockIdx.x picks 64 stocks (one lane per stock, TS rows
+coalesced) and, for PAR levels, blockIdx.y picks a chunk of rows.
+  * every value read through a window keeps a per-lane history ring laid
+    out [slot][lane] (bank-conflict free), slot count a power of two,
+    NaN-initialised -- the reference's "no history before start" rule
+    (Ops.hpp:87-92, 182-186). Rings of values computed in the kernel live
+    in LDS (HBM scratch past the LDS budget); values the kernel reads from
+    HBM (stage inputs, values materialised by a lower level) are ringed in
+    LDS while the budget allows and otherwise re-read from HBM;
+  * a PAR chunk starts `halo` rows before its first stored row so that every
+    window it reads is rebuilt; rings of HBM values are prefilled instead;
+  * HBM values are prefetched one or more rows ahead in registers;
   * window loops run oldest -> newest like CodegenCpp.py:242-270.
 Stream stage kernel (STREAM modules): the same op lowering for one row;
 rings and carried state live in HBM as [slot][S] arrays owned by the stream
 context (the reference's StreamBuffer / StateBuffer, Runtime.cpp:384-546).
 Cross-sectional stages (Rank / Scale) are runtime library kernels."""
-from typing import Dict, List, Tuple
+import os
+from typing import Dict, List, Optional, Tuple
 
-from ..Driver import BUF_INPUT, BUF_OUTPUT, BUF_TEMP, ModuleIR, STAGE_TS
+from ..Driver import BUF_INPUT, BUF_TEMP, ModuleIR, STAGE_TS
 from ..Op import ConstantOp, ForeachBackWindow, Input, OpBase, Output, ReductionOp
+from .plan import PAR, Component, StagePlan, plan_stage
 from .program import literal, program_text
 
 TS_BLOCK = 64
 MAX_STAGE_BUFS = 256
 LDS_LIMIT = 160 * 1024
 # LDS per 64-stock workgroup before history rings spill to HBM scratch
-LDS_BUDGET = 64 * 1024
+LDS_BUDGET = int(os.environ.get("KUN_LDS_BUDGET", 40 * 1024))
 
 _BIN = {"Add": "+", "Sub": "-", "Mul": "*", "Div": "/"}
 _CMP = {"GreaterThan": ">", "GreaterEqual": ">=", "LessThan": "<", "LessEqual": "<=", "Equals": "=="}
@@ -33,6 +40,7 @@ _CONST_BIN = {"AddConst": "+", "SubConst": "-", "MulConst": "*", "DivConst": "/"
               "GreaterThanConst": ">", "LessThanConst": "<"}
 _UNARY = {"Sqrt": "vsqrt", "Abs": "vabs", "Sign": "vsign", "Not": "bnot", "Exp": "exp_fast"}
 _BINFN = {"Max": "vmax", "Min": "vmin", "Or": "bor", "And": "band"}
+_LAG_OPS = ("BackRef", "FastWindowedSum", "WindowedLinearRegression")
 
 
 def c_literal(v, T: str) -> str:
@@ -61,120 +69,36 @@ def ident(name: str) -> str:
     return "".join(c if c.isalnum() else "_" for c in name)
 
 
-class StageEmitter:
-    """Emits the batch kernel and (for STREAM modules) the tick/init kernels
-    of one time-series stage."""
+def _window_reads(op) -> Optional[Tuple[OpBase, int]]:
+    """(source, largest offset) of an op reading a value through a window."""
+    name = op.__class__.__name__
+    if name == "IterValue":
+        return op.inputs[1], op.inputs[0].attrs["window"] - 1
+    if name in _LAG_OPS:
+        return op.inputs[0], op.attrs["window"]
+    return None
 
-    def __init__(self, module: ModuleIR, stage, kname: str):
+
+class _Lowering:
+    """Per-row lowering of stage ops to HIP statements, shared by the batch
+    component kernels and the stream tick kernels. Subclasses provide the
+    storage hooks: win(), input_expr(), output_lines(), after_value(),
+    lane_expr() and tmem()."""
+
+    time_idx = "i"
+
+    def _setup(self, module: ModuleIR, stage):
         self.m = module
         self.st = stage
-        self.kname = kname
-        self.stream = module.stream_mode
         self.T = "float" if module.dtype == "float" else "double"
         self.esz = 4 if self.T == "float" else 8
-        self.ops: List[OpBase] = stage.func.ops
-        self.idx: Dict[OpBase, int] = {op: i for i, op in enumerate(self.ops)}
+        self.idx: Dict[OpBase, int] = {op: i for i, op in enumerate(stage.func.ops)}
         self.slot: Dict[int, int] = {}
         for b in list(stage.in_bufs) + list(stage.out_bufs):
             if b.idx not in self.slot:
                 self.slot[b.idx] = len(self.slot)
         if len(self.slot) > MAX_STAGE_BUFS:
             raise RuntimeError(f"stage {stage.name} uses {len(self.slot)} buffers (limit {MAX_STAGE_BUFS})")
-        # storage plan: T slots and int slots; LDS rows in batch, HBM rows in stream
-        self.ring: Dict[int, Tuple[int, int]] = {}      # op -> (ring slots, first T slot)
-        self.tslot: Dict[int, int] = {}                 # op -> first T slot of its state
-        self.islot: Dict[int, int] = {}                 # op -> first int slot of its state
-        self.n_t = 0
-        self.n_i = 0
-        self._plan()
-        # batch kernels: the first lds_t T slots (and all int slots when they
-        # fit) live in LDS, the rest in per-stage HBM scratch [slot][S]
-        if self.stream:
-            self.lds_t, self.lds_i = 0, 0
-        else:
-            # rings first, in slot order, while they fit the LDS budget; the
-            # sorted windows (and their indices) always live in HBM scratch
-            self.lds_i = 0
-            cap = LDS_BUDGET // (self.esz * TS_BLOCK)
-            self.lds_t = 0
-            for lo, hi in sorted((base, base + slots) for slots, base in self.ring.values()):
-                if hi <= cap:
-                    self.lds_t = hi
-                else:
-                    break
-
-    def _plan(self):
-        maxoff: Dict[int, int] = {}
-        for op in self.ops:
-            name = op.__class__.__name__
-            if name == "IterValue":
-                src, off = op.inputs[1], op.inputs[0].attrs["window"] - 1
-            elif name in ("BackRef", "FastWindowedSum", "WindowedLinearRegression"):
-                src, off = op.inputs[0], op.attrs["window"]
-            else:
-                continue
-            k = self.idx[src]
-            maxoff[k] = max(maxoff.get(k, 0), off)
-        for k in sorted(maxoff):
-            slots = _pow2(maxoff[k] + 1)
-            self.ring[k] = (slots, self.n_t)
-            self.n_t += slots
-        for op in self.ops:
-            k = self.idx[op]
-            name = op.__class__.__name__
-            if name == "SkipListState":
-                w = op.attrs["window"]
-                self.tslot[k] = self.n_t
-                self.n_t += w + 1
-                self.islot[k] = self.n_i
-                self.n_i += w + 1 + (2 if self.stream else 0)
-            elif self.stream:
-                nt, ni = {"FastWindowedSum": (3, 1), "ExpMovingAvg": (1, 0), "Accumulator": (1, 0),
-                          "WindowedLinearRegression": (6, 1)}.get(name, (0, 0))
-                if nt or ni:
-                    self.tslot[k] = self.n_t
-                    self.islot[k] = self.n_i
-                    self.n_t += nt
-                    self.n_i += ni
-
-    @property
-    def lds_bytes(self) -> int:
-        return 0 if self.stream else (self.lds_t * self.esz + self.lds_i * 4) * TS_BLOCK
-
-    @property
-    def state_bytes(self) -> int:
-        """Per-stock HBM bytes: carried state (stream) or spilled rings (batch)."""
-        if self.stream:
-            return self.n_t * self.esz + self.n_i * 4
-        return (self.n_t - self.lds_t) * self.esz + (self.n_i - self.lds_i) * 4
-
-    # -- addressing ------------------------------------------------------
-    def tmem(self, slot, base: int = None) -> str:
-        """T storage row `slot` of this lane; `base` is the first slot of the
-        object the row belongs to (objects never straddle LDS and HBM)."""
-        if self.stream:
-            return f"stT[({slot}) * A.S + s]"
-        b = base if base is not None else (slot if isinstance(slot, int) else 0)
-        if b < self.lds_t:
-            return f"lds[(({slot}) << 6) + lane]"
-        return f"stT[(({slot}) - {self.lds_t}) * A.S + s]"
-
-    def imem(self, slot) -> str:
-        if self.stream:
-            return f"istp[({slot}) * A.S + s]"
-        if self.lds_i:
-            return f"ldsi[(({slot}) << 6) + lane]"
-        return f"istp[({slot}) * A.S + s]"
-
-    def ring_read(self, k: int, off) -> str:
-        slots, base = self.ring[k]
-        return self.tmem(f"{base} + (((int)i - ({off})) & {slots - 1})", base)
-
-    def ring_push(self, k: int) -> List[str]:
-        if k not in self.ring:
-            return []
-        slots, base = self.ring[k]
-        return [f"{self.tmem(f'{base} + ((int)i & {slots - 1})', base)} = v{k};"]
 
     def v(self, op) -> str:
         return f"v{self.idx[op]}"
@@ -186,6 +110,19 @@ class StageEmitter:
         lay = self.m.input_layout if b.kind == BUF_INPUT else self.m.output_layout
         return "1" if lay == "TS" else "0"
 
+    def ring_read(self, k: int, off) -> str:
+        slots, base = self.ring[k]
+        return self.tmem(f"{base} + (((int)i - ({off})) & {slots - 1})", base)
+
+    def ring_push(self, k: int) -> List[str]:
+        if k not in self.ring:
+            return []
+        slots, base = self.ring[k]
+        return [f"{self.tmem(f'{base} + ((int)i & {slots - 1})', base)} = v{k};"]
+
+    def after_value(self, k: int) -> List[str]:
+        return self.ring_push(k)
+
     # -- ops -------------------------------------------------------------
     def op_lines(self, op) -> List[str]:
         T = self.T
@@ -194,22 +131,11 @@ class StageEmitter:
         ins = [self.v(i) for i in op.inputs]
         out = f"v{k}"
         L: List[str] = []
-        time_idx = "0" if self.stream else "i"
+        time_idx = self.time_idx
         if isinstance(op, Input):
-            b = self.m.buffer(op.attrs["name"])
-            p = f"b{b.idx}"
-            if op.attrs.get("single_value", False) or self.stream:
-                L.append(f"const {T} {out} = {p}[s];")
-            elif b.kind == BUF_INPUT:
-                L.append(f"const {T} {out} = {p}[kun::at<{self.buf_layout(op)}>(A.start + i, s, A.S, A.T, A.blk)];")
-            else:
-                L.append(f"const {T} {out} = {p}[kun::at<{self.buf_layout(op)}>(i, s, A.S, A.L, A.blk)];")
+            L.append(f"const {T} {out} = {self.input_expr(op)};")
         elif isinstance(op, Output):
-            b = self.m.buffer(op.attrs["name"])
-            if self.stream:
-                L.append(f"b{b.idx}[s] = {ins[0]};")
-            else:
-                L.append(f"b{b.idx}[kun::at<{self.buf_layout(op)}>(i, s, A.S, A.L, A.blk)] = {ins[0]};")
+            L.extend(self.output_lines(op, ins[0]))
             L.append(f"const {T} {out} = {ins[0]};")
         elif name in ("WindowedTempOutput", "ReturnFirstValue"):
             L.append(f"const {T} {out} = {ins[0]};")
@@ -238,14 +164,14 @@ class StageEmitter:
         elif name == "Select":
             L.append(f"const {T} {out} = kun::sel({ins[0]}, {ins[1]}, {ins[2]});")
         elif name == "IterValue":
-            L.append(f"const {T} {out} = {self.ring_read(self.idx[op.inputs[1]], 'iter')};")
+            L.append(f"const {T} {out} = {self.win(op.inputs[1], 'iter')};")
         elif name == "WindowLoopIndex":
             w = op.inputs[0].attrs["window"]
             L.append(f"const {T} {out} = ({T})({w} - 1 - iter);")
         elif name == "BackRef":
-            L.append(f"const {T} {out} = {self.ring_read(self.idx[op.inputs[0]], op.attrs['window'])};")
+            L.append(f"const {T} {out} = {self.win(op.inputs[0], op.attrs['window'])};")
         elif name == "FastWindowedSum":
-            old = self.ring_read(self.idx[op.inputs[0]], op.attrs["window"])
+            old = self.win(op.inputs[0], op.attrs["window"])
             L.append(f"const {T} {out} = st{k}.step({old}, {ins[0]});")
         elif name == "ExpMovingAvg":
             w = op.attrs["window"]
@@ -258,7 +184,7 @@ class StageEmitter:
             L.append(f"acc{a} = kun::sign_set({ins[1]}) ? {ins[2]} : {ins[0]};")
             L.append(f"const {T} {out} = acc{a};")
         elif name == "WindowedLinearRegression":
-            old = self.ring_read(self.idx[op.inputs[0]], op.attrs["window"])
+            old = self.win(op.inputs[0], op.attrs["window"])
             L.append(f"st{k}.step({old}, {ins[0]}, ({T}){op.attrs['window']});")
             L.append(f"const {T} {out} = 0;")
         elif name.startswith("WindowedLinearRegression") and name.endswith("Impl"):
@@ -291,14 +217,8 @@ class StageEmitter:
             raise AssertionError("handled by the loop emitter")
         else:
             raise RuntimeError(f"HIP emitter: unsupported op {name}")
-        L.extend(self.ring_push(k))
+        L.extend(self.after_value(k))
         return L
-
-    def lane_expr(self, k: int = None) -> str:
-        """Column of this lane in sorted-window storage of op k."""
-        if self.stream or (k is not None and self.tslot.get(k, 0) >= self.lds_t):
-            return "s"
-        return "lane"
 
     def loop_lines(self, loop) -> List[str]:
         T = self.T
@@ -370,13 +290,13 @@ class StageEmitter:
                 L.append(f"v{k} = r{k}_i;")
             elif name == "ReduceRank":
                 L.append(f"v{k} = r{k}_lt + (r{k}_eq + {T}(1.0)) / {T}(2.0);")
-            L.extend(self.ring_push(k))
+            L.extend(self.after_value(k))
         return L
 
-    def step_lines(self) -> List[str]:
+    def step_lines(self, skip=()) -> List[str]:
         L = []
         for op in self.ops:
-            if op.get_parent() is not None or isinstance(op, ReductionOp):
+            if op.get_parent() is not None or isinstance(op, ReductionOp) or op in skip:
                 continue
             L.extend(self.loop_lines(op) if isinstance(op, ForeachBackWindow) else self.op_lines(op))
         return L
@@ -391,10 +311,77 @@ class StageEmitter:
                     continue
                 seen.add(b.idx)
                 const = "const " if isinstance(op, Input) else ""
-                L.append(f"  {const}{T} *__restrict__ b{b.idx} = ({const}{T} *){args}.bufs[{self.slot[b.idx]}];")
+                L.append(f"{const}{T} *__restrict__ b{b.idx} = ({const}{T} *){args}.bufs[{self.slot[b.idx]}];")
         return L
 
-    # state declarations: batch = fresh locals; stream = loaded from HBM
+
+class StageEmitter(_Lowering):
+    """Tick and init kernels of one time-series stage of a STREAM module:
+    rings and carried state are [slot][S] arrays in HBM."""
+
+    time_idx = "0"
+
+    def __init__(self, module: ModuleIR, stage, kname: str):
+        assert module.stream_mode
+        self._setup(module, stage)
+        self.kname = kname
+        self.ops: List[OpBase] = stage.func.ops
+        self.ring: Dict[int, Tuple[int, int]] = {}      # op -> (ring slots, first T slot)
+        self.tslot: Dict[int, int] = {}                 # op -> first T slot of its state
+        self.islot: Dict[int, int] = {}                 # op -> first int slot of its state
+        self.n_t = 0
+        self.n_i = 0
+        maxoff: Dict[int, int] = {}
+        for op in self.ops:
+            wr = _window_reads(op)
+            if wr:
+                k = self.idx[wr[0]]
+                maxoff[k] = max(maxoff.get(k, 0), wr[1])
+        for k in sorted(maxoff):
+            slots = _pow2(maxoff[k] + 1)
+            self.ring[k] = (slots, self.n_t)
+            self.n_t += slots
+        for op in self.ops:
+            k = self.idx[op]
+            name = op.__class__.__name__
+            if name == "SkipListState":
+                w = op.attrs["window"]
+                self.tslot[k] = self.n_t
+                self.n_t += w + 1
+                self.islot[k] = self.n_i
+                self.n_i += w + 3
+            else:
+                nt, ni = {"FastWindowedSum": (3, 1), "ExpMovingAvg": (1, 0), "Accumulator": (1, 0),
+                          "WindowedLinearRegression": (6, 1)}.get(name, (0, 0))
+                if nt or ni:
+                    self.tslot[k] = self.n_t
+                    self.islot[k] = self.n_i
+                    self.n_t += nt
+                    self.n_i += ni
+
+    @property
+    def state_bytes(self) -> int:
+        """Per-stock HBM bytes of carried state."""
+        return self.n_t * self.esz + self.n_i * 4
+
+    def tmem(self, slot, base: int = None) -> str:
+        return f"stT[({slot}) * A.S + s]"
+
+    def imem(self, slot) -> str:
+        return f"istp[({slot}) * A.S + s]"
+
+    def lane_expr(self, k: int = None) -> str:
+        return "s"
+
+    def win(self, src, off) -> str:
+        return self.ring_read(self.idx[src], off)
+
+    def input_expr(self, op) -> str:
+        return f"b{self.m.buffer(op.attrs['name']).idx}[s]"
+
+    def output_lines(self, op, val: str) -> List[str]:
+        return [f"b{self.m.buffer(op.attrs['name']).idx}[s] = {val};"]
+
     def state_decls(self) -> List[str]:
         T = self.T
         L = []
@@ -404,40 +391,23 @@ class StageEmitter:
             ts, is_ = self.tslot.get(k), self.islot.get(k)
             if name == "FastWindowedSum":
                 L.append(f"  kun::FastSum<{T}> st{k};")
-                if self.stream:
-                    L.append(f"  st{k}.v = {self.tmem(ts)}; st{k}.cadd = {self.tmem(ts + 1)}; "
-                             f"st{k}.csub = {self.tmem(ts + 2)}; st{k}.nnan = {self.imem(is_)};")
-                else:
-                    L.append(f"  st{k}.init({op.attrs['window']});")
+                L.append(f"  st{k}.v = {self.tmem(ts)}; st{k}.cadd = {self.tmem(ts + 1)}; "
+                         f"st{k}.csub = {self.tmem(ts + 2)}; st{k}.nnan = {self.imem(is_)};")
             elif name == "WindowedLinearRegression":
                 L.append(f"  kun::LinReg<{T}> st{k};")
-                if self.stream:
-                    fields = ["is", "xs", "x2", "ys", "y2", "xy"]
-                    L.append("  " + " ".join(f"st{k}.{f} = {self.tmem(ts + j)};" for j, f in enumerate(fields))
-                             + f" st{k}.nnan = {self.imem(is_)};")
-                else:
-                    L.append(f"  st{k}.init({op.attrs['window']});")
+                fields = ["is", "xs", "x2", "ys", "y2", "xy"]
+                L.append("  " + " ".join(f"st{k}.{f} = {self.tmem(ts + j)};" for j, f in enumerate(fields))
+                         + f" st{k}.nnan = {self.imem(is_)};")
             elif name == "ExpMovingAvg":
                 L.append(f"  kun::Ema<{T}> st{k};")
-                if self.stream:
-                    L.append(f"  st{k}.v = {self.tmem(ts)};")
-                elif len(op.inputs) > 1:
-                    ib = self.m.buffer(op.inputs[1].attrs["name"]).idx
-                    L.append(f"  st{k}.v = (({T} *)A.bufs[{self.slot[ib]}])[s];")
-                else:
-                    L.append(f"  st{k}.v = kun::qnan<{T}>();")
+                L.append(f"  st{k}.v = {self.tmem(ts)};")
             elif name == "Accumulator":
-                L.append(f"  {T} acc{k} = {self.tmem(ts) if self.stream else '0'};")
+                L.append(f"  {T} acc{k} = {self.tmem(ts)};")
             elif name == "SkipListState":
                 w = op.attrs["window"]
-                if self.stream:
-                    L.append(f"  kun::SortedWin<{T}, {w}> st{k}; st{k}.init(stT + (size_t){ts} * A.S, "
-                             f"istp + (size_t){is_} * A.S, A.S);")
-                    L.append(f"  st{k}.n = {self.imem(is_ + w + 1)}; st{k}.last = {self.imem(is_ + w + 2)};")
-                else:
-                    assert ts >= self.lds_t and not self.lds_i
-                    L.append(f"  kun::SortedWin<{T}, {w}> st{k}; st{k}.init(stT + (size_t)({ts - self.lds_t}) * A.S, "
-                             f"istp + (size_t){is_} * A.S, A.S);")
+                L.append(f"  kun::SortedWin<{T}, {w}> st{k}; st{k}.init(stT + (size_t){ts} * A.S, "
+                         f"istp + (size_t){is_} * A.S, A.S);")
+                L.append(f"  st{k}.n = {self.imem(is_ + w + 1)}; st{k}.last = {self.imem(is_ + w + 2)};")
         return L
 
     def state_stores(self) -> List[str]:
@@ -462,32 +432,6 @@ class StageEmitter:
                 L.append(f"  {self.imem(is_ + w + 1)} = st{k}.n; {self.imem(is_ + w + 2)} = st{k}.last;")
         return L
 
-    # -- kernels ---------------------------------------------------------
-    def batch_kernel(self) -> str:
-        T = self.T
-        if self.lds_bytes > LDS_LIMIT:
-            raise RuntimeError(f"stage {self.st.name}: {self.lds_bytes} bytes of LDS exceed the gfx950 limit")
-        L = [f'extern "C" __global__ __launch_bounds__({TS_BLOCK}) void {self.kname}(kun::StageArgs A) {{',
-             "  const int lane = threadIdx.x;",
-             f"  const long s = (long)blockIdx.x * {TS_BLOCK} + lane;"]
-        if self.lds_t:
-            L.append(f"  __shared__ {T} lds[{self.lds_t * TS_BLOCK}];")
-        if self.lds_i:
-            L.append(f"  __shared__ int32_t ldsi[{self.lds_i * TS_BLOCK}];")
-        L.append("  if (s >= A.S) return;")
-        if self.n_t > self.lds_t or self.n_i > self.lds_i:
-            L.append(f"  {T} *__restrict__ stT = ({T} *)A.scratch;")
-            L.append(f"  int32_t *__restrict__ istp = (int32_t *)(stT + (size_t){self.n_t - self.lds_t} * A.S);")
-        L.extend(self.buffer_ptrs())
-        for k, (slots, base) in sorted(self.ring.items()):
-            L.append(f"  for (int q = 0; q < {slots}; ++q) {self.tmem(f'{base} + q', base)} = kun::qnan<{T}>();")
-        L.extend(self.state_decls())
-        L.append("  for (long i = 0; i < A.L; ++i) {")
-        L.extend("    " + x for x in self.step_lines())
-        L.append("  }")
-        L.append("}")
-        return "\n".join(L)
-
     def stream_kernels(self) -> str:
         T = self.T
         tick = [f'extern "C" __global__ __launch_bounds__({TS_BLOCK}) void {self.kname}_tick(kun::StreamArgs A) {{',
@@ -496,7 +440,7 @@ class StageEmitter:
                 f"  {T} *__restrict__ stT = ({T} *)A.st;",
                 f"  int32_t *__restrict__ istp = (int32_t *)(stT + (size_t){self.n_t} * A.S);",
                 "  const long i = A.tick;"]
-        tick.extend(self.buffer_ptrs())
+        tick.extend("  " + x for x in self.buffer_ptrs())
         tick.extend(self.state_decls())
         tick.append("  {")
         tick.extend("    " + x for x in self.step_lines())
@@ -535,24 +479,397 @@ class StageEmitter:
         return "\n".join(tick) + "\n\n" + "\n".join(init)
 
 
-def emit_module(module: ModuleIR) -> str:
-    """Full HIP translation unit for one module: kernels + descriptor."""
+class CompEmitter(_Lowering):
+    """One component of one level of a batch time-series stage.
+
+    Row sources are the values this component reads from HBM: stage inputs
+    (Input ops) and values materialised by lower levels. They are loaded
+    `prefetch` rows ahead. Window reads of a row source use an LDS ring when
+    the budget allows, a prefetched lag load for single-offset reads
+    (BackRef, FastWindowedSum, regression sums), or a direct HBM load."""
+
+    def __init__(self, module: ModuleIR, stage, plan: StagePlan, comp: Component, kind: int,
+                 lds_budget: int = LDS_BUDGET):
+        self._setup(module, stage)
+        self.plan = plan
+        self.kind = kind
+        self.ext = list(comp.ext)
+        self.ops: List[OpBase] = list(comp.ops)
+        opset = set(self.ops)
+        self.computed = {op for op in self.ops if not isinstance(op, (Input, ConstantOp))}
+        self.single = {op for op in self.ops if isinstance(op, Input) and op.attrs.get("single_value", False)}
+        srcs = [op for op in self.ops if isinstance(op, Input) and op not in self.single] + self.ext
+        self.srcs = sorted(srcs, key=lambda o: self.idx[o])
+        srcset = set(self.srcs)
+        # window reads
+        ring_need: Dict[OpBase, int] = {}   # computed value -> largest offset
+        src_iter: Dict[OpBase, int] = {}    # row source read by IterValue -> largest offset
+        self.lags: Dict[Tuple[int, int], None] = {}  # (source op idx, offset) read through a lag load
+        for op in self.ops:
+            wr = _window_reads(op)
+            if not wr:
+                continue
+            src, off = wr
+            if src in srcset:
+                if op.__class__.__name__ == "IterValue":
+                    src_iter[src] = max(src_iter.get(src, 0), off)
+                else:
+                    self.lags[(self.idx[src], off)] = None
+            else:
+                assert src in opset, "window read of a value outside the component"
+                ring_need[src] = max(ring_need.get(src, 0), off)
+        self.ring: Dict[int, Tuple[int, int]] = {}
+        self.tslot: Dict[int, int] = {}
+        self.islot: Dict[int, int] = {}
+        cap = lds_budget // (self.esz * TS_BLOCK)
+        n_lds = 0
+        spilled = []
+        for op in sorted(ring_need, key=lambda o: self.idx[o]):
+            slots = _pow2(ring_need[op] + 1)
+            if n_lds + slots <= cap:
+                self.ring[self.idx[op]] = (slots, n_lds)
+                n_lds += slots
+            else:
+                spilled.append((op, slots))
+        self.src_ring: Dict[int, int] = {}  # row source idx -> largest offset, ringed in LDS
+        self.direct: set = set()            # row sources read by IterValue straight from HBM
+        for op in sorted(src_iter, key=lambda o: (-src_iter[o], self.idx[o])):
+            slots = _pow2(src_iter[op] + 1)
+            if n_lds + slots <= cap:
+                self.ring[self.idx[op]] = (slots, n_lds)
+                self.src_ring[self.idx[op]] = src_iter[op]
+                n_lds += slots
+            else:
+                self.direct.add(self.idx[op])
+        self.lds_t = n_lds
+        n_t = n_lds
+        for op, slots in spilled:
+            self.ring[self.idx[op]] = (slots, n_t)
+            n_t += slots
+        n_i = 0
+        for op in self.ops:
+            if op.__class__.__name__ == "SkipListState":
+                w = op.attrs["window"]
+                self.tslot[self.idx[op]] = n_t
+                n_t += w + 1
+                self.islot[self.idx[op]] = n_i
+                n_i += w + 1
+        self.n_t, self.n_i = n_t, n_i
+        self.halo = self._halo() if kind == PAR else 0
+        nops = len(self.computed)
+        has_loop = any(isinstance(op, ForeachBackWindow) for op in self.ops)
+        if kind == PAR or has_loop or nops > 60:
+            self.prefetch = 1
+        elif nops > 24:
+            self.prefetch = 4
+        else:
+            self.prefetch = 8
+
+    # -- planning helpers -------------------------------------------------
+    def _halo(self) -> int:
+        """Rows of history a PAR chunk recomputes before its first stored row:
+        the longest chain of in-component window offsets."""
+        need: Dict[OpBase, int] = {op: 0 for op in self.ops}
+        for op in reversed(self.ops):
+            if op not in self.computed:
+                continue
+            n = need[op]
+            name = op.__class__.__name__
+            for j, inp in enumerate(op.inputs):
+                if inp not in self.computed:
+                    continue
+                if name == "IterValue" and j == 1:
+                    add = op.inputs[0].attrs["window"] - 1
+                elif name in _LAG_OPS and j == 0:
+                    add = op.attrs["window"]
+                elif isinstance(op, ForeachBackWindow):
+                    add = op.attrs["window"] - 1
+                else:
+                    add = 0
+                need[inp] = max(need[inp], n + add)
+        return max((need[op] for op in self.computed), default=0)
+
+    @property
+    def lds_bytes(self) -> int:
+        return self.lds_t * self.esz * TS_BLOCK
+
+    @property
+    def spill_bytes(self) -> int:
+        """HBM bytes per (stock, chunk) column: spilled rings and sorted windows."""
+        return (self.n_t - self.lds_t) * self.esz + self.n_i * 4
+
+    # -- storage hooks ----------------------------------------------------
+    def tmem(self, slot, base: int = None) -> str:
+        b = base if base is not None else (slot if isinstance(slot, int) else 0)
+        if b < self.lds_t:
+            return f"lds[(({slot}) << 6) + lane]"
+        return f"stT[(size_t)(({slot}) - {self.lds_t}) * cst + col]"
+
+    def imem(self, slot) -> str:
+        return f"istp[(size_t)({slot}) * cst + col]"
+
+    def lane_expr(self, k: int = None) -> str:
+        return "col"
+
+    def src_load(self, op, row: str) -> str:
+        """HBM load of row source `op` at run row `row` (0 <= row < L)."""
+        k = self.idx[op]
+        if isinstance(op, Input):
+            b = self.m.buffer(op.attrs["name"])
+            lay = self.buf_layout(op)
+            if b.kind == BUF_INPUT:
+                return f"b{b.idx}[kun::at<{lay}>(A.start + ({row}), s, A.S, A.T, A.blk)]"
+            return f"b{b.idx}[kun::at<{lay}>(({row}), s, A.S, A.L, A.blk)]"
+        return f"m{k}[(size_t)({row}) * A.S + s]"
+
+    def lag_load(self, op, row: str) -> str:
+        return f"(({row}) >= 0 ? {self.src_load(op, row)} : kun::qnan<{self.T}>())"
+
+    def win(self, src, off) -> str:
+        k = self.idx[src]
+        if src in self.computed:
+            return self.ring_read(k, off)
+        if off != "iter":
+            # single-offset read of a row source: its prefetched lag load (a
+            # ring kept for the source's window loops may be shorter)
+            return f"lg{k}_{off}"
+        if k in self.src_ring:
+            return self.ring_read(k, off)
+        return self.lag_load(src, f"i - ({off})")
+
+    def input_expr(self, op) -> str:
+        raise AssertionError("row sources are loaded at the top of the row")
+
+    def store_guard(self) -> str:
+        return "if (i >= c0) " if self.kind == PAR else ""
+
+    def output_lines(self, op, val: str) -> List[str]:
+        b = self.m.buffer(op.attrs["name"])
+        return [f"{self.store_guard()}b{b.idx}[kun::at<{self.buf_layout(op)}>(i, s, A.S, A.L, A.blk)] = {val};"]
+
+    def after_value(self, k: int) -> List[str]:
+        L = self.ring_push(k)
+        if self.st.func.ops[k] in self.plan.mat:
+            L.append(f"{self.store_guard()}w{k}[(size_t)i * A.S + s] = v{k};")
+        return L
+
+    # -- code -------------------------------------------------------------
+    def state_decls(self) -> List[str]:
+        T = self.T
+        L = []
+        for op in self.ops:
+            k = self.idx[op]
+            name = op.__class__.__name__
+            if name == "FastWindowedSum":
+                L.append(f"kun::FastSum<{T}> st{k}; st{k}.init({op.attrs['window']});")
+            elif name == "WindowedLinearRegression":
+                L.append(f"kun::LinReg<{T}> st{k}; st{k}.init({op.attrs['window']});")
+            elif name == "ExpMovingAvg":
+                L.append(f"kun::Ema<{T}> st{k};")
+                if len(op.inputs) > 1:
+                    ib = self.m.buffer(op.inputs[1].attrs["name"]).idx
+                    L.append(f"st{k}.v = ((const {T} *)A.bufs[{self.slot[ib]}])[s];")
+                else:
+                    L.append(f"st{k}.v = kun::qnan<{T}>();")
+            elif name == "Accumulator":
+                L.append(f"{T} acc{k} = 0;")
+            elif name == "SkipListState":
+                w = op.attrs["window"]
+                ts, is_ = self.tslot[k], self.islot[k]
+                L.append(f"kun::SortedWin<{T}, {w}> st{k}; st{k}.init(stT + (size_t)({ts - self.lds_t}) * cst, "
+                         f"istp + (size_t){is_} * cst, cst);")
+        return L
+
+    def _src_row_used(self, op) -> bool:
+        """True when row source `op` is needed at the current row: read
+        elementwise, or pushed into its LDS ring."""
+        if self.idx[op] in self.src_ring:
+            return True
+        for o in self.ops:
+            if isinstance(o, ForeachBackWindow):
+                continue
+            for j, inp in enumerate(o.inputs):
+                if inp is not op:
+                    continue
+                name = o.__class__.__name__
+                if (name == "IterValue" and j == 1) or (name == "BackRef" and j == 0):
+                    continue
+                return True
+        return False
+
+    def row_lines(self, q: int, P: int) -> List[str]:
+        """Statements of one row `i` using prefetch slot q."""
+        T = self.T
+        L = []
+        nxt = f"(i + {P} < c1 ? i + {P} : c1 - 1)"
+        for op in self.srcs:
+            k = self.idx[op]
+            if not self._src_row_used(op):
+                continue
+            L.append(f"const {T} v{k} = pf{q}_{k}; pf{q}_{k} = {self.src_load(op, nxt)};")
+            if k in self.src_ring:
+                L.extend(self.ring_push(k))
+        for (k, off) in self.lags:
+            op = self.st.func.ops[k]
+            L.append(f"const {T} lg{k}_{off} = pl{q}_{k}_{off}; "
+                     f"pl{q}_{k}_{off} = {self.lag_load(op, f'{nxt} - {off}')};")
+        L.extend(self.step_lines(skip=set(self.srcs) | self.single))
+        return L
+
+    def body(self, zoff: int) -> List[str]:
+        """Statements of the component, run by one lane for one stock."""
+        T = self.T
+        L = []
+        if self.kind == PAR:
+            L.append("const long c0 = (long)blockIdx.y * A.chunk;")
+            L.append("const long c1 = c0 + A.chunk < A.L ? c0 + A.chunk : A.L;")
+            L.append(f"const long rbeg = c0 - {self.halo} > 0 ? c0 - {self.halo} : 0;")
+        else:
+            L.append("const long c0 = 0, c1 = A.L, rbeg = 0;")
+            L.append("(void)c0;")
+        if self.n_t > self.lds_t or self.n_i:
+            L.append("const long col = (long)blockIdx.y * A.S + s;")
+            L.append("const long cst = A.S * (long)gridDim.y;")
+            L.append(f"{T} *__restrict__ stT = ({T} *)((char *)A.scratch + (size_t){zoff} * cst);")
+            L.append(f"int32_t *__restrict__ istp = (int32_t *)(stT + (size_t){self.n_t - self.lds_t} * cst);")
+        L.extend(self.buffer_ptrs())
+        rows = "(size_t)A.L * A.S"
+        for op in self.ext:
+            k = self.idx[op]
+            L.append(f"const {T} *__restrict__ m{k} = (const {T} *)A.mat + (size_t){self.plan.mat[op]} * {rows};")
+        for op in self.ops:
+            if op in self.computed and op in self.plan.mat:
+                k = self.idx[op]
+                L.append(f"{T} *__restrict__ w{k} = ({T} *)A.mat + (size_t){self.plan.mat[op]} * {rows};")
+        for op in sorted(self.single, key=lambda o: self.idx[o]):
+            b = self.m.buffer(op.attrs["name"])
+            L.append(f"const {T} v{self.idx[op]} = b{b.idx}[s];")
+        for k, (slots, base) in sorted(self.ring.items()):
+            L.append(f"for (int q = 0; q < {slots}; ++q) {self.tmem(f'{base} + q', base)} = kun::qnan<{T}>();")
+        if self.kind == PAR:
+            for k, off in sorted(self.src_ring.items()):
+                slots, base = self.ring[k]
+                op = self.st.func.ops[k]
+                L.append(f"for (long r = rbeg - {off} > 0 ? rbeg - {off} : 0; r < rbeg; ++r) "
+                         f"{self.tmem(f'{base} + ((int)r & {slots - 1})', base)} = {self.src_load(op, 'r')};")
+        L.extend(self.state_decls())
+        P = self.prefetch
+        for q in range(P):
+            first = f"(rbeg + {q} < c1 ? rbeg + {q} : c1 - 1)"
+            for op in self.srcs:
+                if self._src_row_used(op):
+                    L.append(f"{T} pf{q}_{self.idx[op]} = {self.src_load(op, first)};")
+            for (k, off) in self.lags:
+                L.append(f"{T} pl{q}_{k}_{off} = {self.lag_load(self.st.func.ops[k], f'{first} - {off}')};")
+        if P == 1:
+            L.append("for (long i = rbeg; i < c1; ++i) {")
+            L.extend("  " + x for x in self.row_lines(0, 1))
+            L.append("}")
+        else:
+            L.append(f"for (long i0 = rbeg; i0 < c1; i0 += {P}) {{")
+            for q in range(P):
+                L.append(f"  {{ const long i = i0 + {q};")
+                L.append("  if (i < c1) {")
+                L.extend("    " + x for x in self.row_lines(q, P))
+                L.append("  } }")
+            L.append("}")
+        return L
+
+
+class LevelKernel:
+    """One kernel of a batch stage: the components of one plan level."""
+
+    def __init__(self, module: ModuleIR, stage, plan: StagePlan, level, kname: str, lds_budget: int = None):
+        self.kname = kname
+        self.kind = level.kind
+        budget = LDS_BUDGET if lds_budget is None else lds_budget
+        self.comps = [CompEmitter(module, stage, plan, c, level.kind, budget) for c in level.comps]
+        self.T = self.comps[0].T
+        self.zoff = []
+        off = 0
+        for c in self.comps:
+            self.zoff.append(off)
+            off += c.spill_bytes
+        self.spill_bytes = off
+        self.halo = max(c.halo for c in self.comps)
+        self.lds_bytes = max(c.lds_bytes for c in self.comps)
+        # algorithmic traffic: each distinct row source read once, each
+        # stored value (outputs, materialised values) written once
+        io = set()
+        for c in self.comps:
+            for op in c.srcs:
+                io.add(("r", op.attrs["name"]) if isinstance(op, Input) else ("m", c.idx[op]))
+            for op in c.ops:
+                if isinstance(op, Output):
+                    io.add(("w", op.attrs["name"]))
+                elif op in c.computed and op in plan.mat:
+                    io.add(("m", c.idx[op]))
+        self.io_values = len(io)
+        if self.lds_bytes > LDS_LIMIT:
+            raise RuntimeError(f"stage {stage.name}: {self.lds_bytes} bytes of LDS exceed the gfx950 limit")
+
+    def source(self) -> str:
+        T = self.T
+        L = [f'extern "C" __global__ __launch_bounds__({TS_BLOCK}) void {self.kname}(kun::StageArgs A) {{',
+             "  const int lane = threadIdx.x;",
+             f"  const long s = (long)blockIdx.x * {TS_BLOCK} + lane;"]
+        lds_t = max(c.lds_t for c in self.comps)
+        if lds_t:
+            L.append(f"  __shared__ {T} lds[{lds_t * TS_BLOCK}];")
+        else:
+            L.append("  (void)lane;")
+        L.append("  if (s >= A.S) return;")
+        multi = len(self.comps) > 1
+        if multi:
+            L.append("  switch (blockIdx.z) {")
+        for z, c in enumerate(self.comps):
+            body = c.body(self.zoff[z])
+            if multi:
+                L.append(f"  case {z}: {{")
+                L.extend("    " + x for x in body)
+                L.append("  } break;")
+            else:
+                L.append("  {")
+                L.extend("    " + x for x in body)
+                L.append("  }")
+        if multi:
+            L.append("  default: break;")
+            L.append("  }")
+        L.append("}")
+        return "\n".join(L)
+
+
+def stage_plan(module: ModuleIR, stage) -> StagePlan:
+    """Kernel plan of a batch stage; KUN_FORCE_SEQ=1 runs every stage as
+    one sequential kernel (a debugging aid)."""
+    return plan_stage(stage.func.ops, force_seq=os.environ.get("KUN_FORCE_SEQ", "") == "1")
+
+
+def emit_module(module: ModuleIR, lds_budget: int = None) -> str:
+    """Full HIP translation unit for one module: kernels + descriptor.
+    `lds_budget` overrides the LDS bytes per workgroup given to history
+    rings before they move to HBM."""
     mid = ident(module.name)
     parts = ['#include "kun_device.hpp"', '#include "kun_module_abi.h"', ""]
     meta = []
     for st in module.stages:
-        if st.kind == STAGE_TS:
-            kname = f"kun_{mid}_s{st.idx}"
+        kname = f"kun_{mid}_s{st.idx}"
+        if st.kind != STAGE_TS:
+            meta.append((st, [], None, None, 0, 0))
+        elif module.stream_mode:
             em = StageEmitter(module, st, kname)
-            if module.stream_mode:
-                parts.append(em.stream_kernels())
-                meta.append((st, None, f"{kname}_tick", f"{kname}_init", 0, em.state_bytes))
-            else:
-                parts.append(em.batch_kernel())
-                meta.append((st, kname, None, None, em.lds_bytes, em.state_bytes))
+            parts.append(em.stream_kernels())
             parts.append("")
+            meta.append((st, [], f"{kname}_tick", f"{kname}_init", em.state_bytes, 0))
         else:
-            meta.append((st, None, None, None, 0, 0))
+            plan = stage_plan(module, st)
+            kerns = []
+            for j, level in enumerate(plan.levels):
+                lk = LevelKernel(module, st, plan, level, f"{kname}_k{j}", lds_budget)
+                parts.append(lk.source())
+                parts.append("")
+                kerns.append(lk)
+            meta.append((st, kerns, None, None, 0, len(plan.mat)))
     kinds = {"INPUT": 0, "OUTPUT": 1, "TEMP": 2}
     lay = {"STs": 0, "TS": 1, "STREAM": 2}
     parts.append(f"static const KunBufferDesc kun_{mid}_bufs[] = {{")
@@ -560,21 +877,27 @@ def emit_module(module: ModuleIR) -> str:
         parts.append(f'  {{"{b.name}", {kinds[b.kind]}, {b.unreliable}, {b.window}}},')
     parts.append("};")
     stage_kind = {"ts": 0, "rank": 1, "scale": 2}
-    for st, *_ in meta:
+    for st, kerns, *_ in meta:
         for tag, arr in (("in", [b.idx for b in st.in_bufs]), ("out", [b.idx for b in st.out_bufs]),
                          ("dep", st.deps)):
             body = ", ".join(str(x) for x in arr) if arr else "-1"
             parts.append(f"static const int kun_{mid}_s{st.idx}_{tag}[] = {{{body}}};")
+        if kerns:
+            parts.append(f"static const KunKernelDesc kun_{mid}_s{st.idx}_kerns[] = {{")
+            for lk in kerns:
+                parts.append(f"  {{(const void *)&{lk.kname}, {lk.kind}, {len(lk.comps)}, {lk.halo}, "
+                             f"{lk.lds_bytes}, {lk.spill_bytes}, {lk.io_values}}},")
+            parts.append("};")
     if module.stream_mode:
         parts.append(f"static const void *kun_{mid}_inits[] = {{"
                      + ", ".join(f"(const void *)&{ini}" if ini else "nullptr" for _, _, _, ini, _, _ in meta) + "};")
     parts.append(f"static const KunStageDesc kun_{mid}_stages[] = {{")
-    for st, kname, tname, ini, lds, sbytes in meta:
-        kern = f"(const void *)&{kname}" if kname else "nullptr"
+    for st, kerns, tname, ini, sbytes, nmat in meta:
         tk = f"(const void *)&{tname}" if tname else "nullptr"
-        parts.append(f'  {{"{st.name}", {stage_kind[st.kind]}, {kern}, {tk}, {TS_BLOCK}, {lds}, {sbytes},'
-                     f" kun_{mid}_s{st.idx}_in, {len(st.in_bufs)}, kun_{mid}_s{st.idx}_out, {len(st.out_bufs)},"
-                     f" kun_{mid}_s{st.idx}_dep, {len(st.deps)}}},")
+        karr = f"kun_{mid}_s{st.idx}_kerns" if kerns else "nullptr"
+        parts.append(f'  {{"{st.name}", {stage_kind[st.kind]}, {karr}, {len(kerns)}, {nmat}, {tk}, {TS_BLOCK},'
+                     f" {sbytes}, kun_{mid}_s{st.idx}_in, {len(st.in_bufs)}, kun_{mid}_s{st.idx}_out,"
+                     f" {len(st.out_bufs)}, kun_{mid}_s{st.idx}_dep, {len(st.deps)}}},")
     parts.append("};")
     prog = program_text(module).replace("\\", "\\\\").replace('"', '\\"').replace("\n", '\\n"\n"')
     parts.append(f'static const char kun_{mid}_program[] = "{prog}";')

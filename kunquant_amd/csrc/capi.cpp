@@ -50,6 +50,20 @@ KUN_API KunExecutorHandle kunCreateMultiThreadExecutor(int numthreads) {
 
 KUN_API void kunDestoryExecutor(KunExecutorHandle ptr) { delete unwrap_exec(ptr); }
 
+KUN_API int kunExecutorSetChunking(KunExecutorHandle exec, long target_workgroups, long halo_factor,
+                                  long min_rows) {
+    return guarded([&] {
+        if (target_workgroups < 1 || halo_factor < 0 || min_rows < 1) throw Error("bad chunking parameters");
+        Executor &ex = **unwrap_exec(exec);
+        std::lock_guard<std::mutex> g(ex.mu);
+        HIPCHECK(hipStreamSynchronize(ex.stream));
+        ex.free_temps();
+        ex.chunking.target_workgroups = target_workgroups;
+        ex.chunking.halo_factor = halo_factor;
+        ex.chunking.min_rows = min_rows;
+    });
+}
+
 KUN_API int kunExecutorSynchronize(KunExecutorHandle exec) {
     return guarded([&] { HIPCHECK(hipStreamSynchronize((*unwrap_exec(exec))->stream)); });
 }
@@ -121,6 +135,25 @@ KUN_API int kunModuleStageNumInputs(KunModuleHandle m, int i) { return unwrap_mo
 KUN_API int kunModuleStageInput(KunModuleHandle m, int i, int j) { return unwrap_mod(m)->stages[i].in[j]; }
 KUN_API int kunModuleStageNumOutputs(KunModuleHandle m, int i) { return unwrap_mod(m)->stages[i].nout; }
 KUN_API int kunModuleStageOutput(KunModuleHandle m, int i, int j) { return unwrap_mod(m)->stages[i].out[j]; }
+KUN_API int kunModuleStageNumKernels(KunModuleHandle m, int i) {
+    const KunStageDesc &sd = unwrap_mod(m)->stages[i];
+    return sd.kind == KUN_STAGE_TS ? sd.nkern : 1;
+}
+KUN_API int kunModuleKernelInfo(KunModuleHandle m, int stage, int kern, int *kind, int *ncomp, int *halo,
+                                int *lds_bytes, int *io_values) {
+    const KunStageDesc &sd = unwrap_mod(m)->stages[stage];
+    if (sd.kind != KUN_STAGE_TS || kern < 0 || kern >= sd.nkern) {
+        set_error("no such stage kernel");
+        return 1;
+    }
+    const KunKernelDesc &kd = sd.kerns[kern];
+    *kind = kd.kind;
+    *ncomp = kd.ncomp;
+    *halo = kd.halo;
+    *lds_bytes = kd.lds_bytes;
+    *io_values = kd.io_values;
+    return 0;
+}
 
 KUN_API int kunExecutorEnableProfiling(KunExecutorHandle exec, int on) {
     return guarded([&] {
@@ -138,7 +171,23 @@ KUN_API int kunExecutorStageTime(KunExecutorHandle exec, KunModuleHandle m, int 
         Executor &ex = **unwrap_exec(exec);
         std::lock_guard<std::mutex> g(ex.mu);
         ex.collect_profile();
-        auto it = ex.stage_ms.find({(const void *)m, stage});
+        *total_ms = 0.0;
+        *count = 0;
+        for (auto &kv : ex.stage_ms) {
+            if (std::get<0>(kv.first) != (const void *)m || std::get<1>(kv.first) != stage) continue;
+            *total_ms += kv.second.first;
+            if (std::get<2>(kv.first) == 0) *count = kv.second.second;
+        }
+    });
+}
+
+KUN_API int kunExecutorKernelTime(KunExecutorHandle exec, KunModuleHandle m, int stage, int kern, double *total_ms,
+                                  long *count) {
+    return guarded([&] {
+        Executor &ex = **unwrap_exec(exec);
+        std::lock_guard<std::mutex> g(ex.mu);
+        ex.collect_profile();
+        auto it = ex.stage_ms.find(std::make_tuple((const void *)m, stage, kern));
         *total_ms = it == ex.stage_ms.end() ? 0.0 : it->second.first;
         *count = it == ex.stage_ms.end() ? 0 : it->second.second;
     });

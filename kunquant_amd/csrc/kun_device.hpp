@@ -335,11 +335,13 @@ template <int LAYOUT> __device__ __forceinline__ size_t at(long row, long s, lon
 // the order of its descriptor (inputs, then outputs).
 struct StageArgs {
     void *bufs[KUN_MAX_STAGE_BUFS];
-    void *scratch;     // HBM rows of history rings that do not fit in LDS
+    void *scratch;     // HBM columns of rings that do not fit in LDS, sorted windows
+    void *mat;         // [slot][L][S] values handed between the kernels of the stage
     long S;            // stocks
     long T;            // rows of the user inputs
     long start;        // first row of user inputs to compute
     long L;            // rows to compute
+    long chunk;        // rows per chunk of a PAR kernel (grid y)
     int blk;           // STs blocking length
 };
 

@@ -8,12 +8,13 @@
 #ifndef KUN_MODULE_ABI_H
 #define KUN_MODULE_ABI_H
 
-#define KUN_MODULE_ABI_VERSION 0x4B554E02u
+#define KUN_MODULE_ABI_VERSION 0x4B554E03u
 
 enum { KUN_BUF_INPUT = 0, KUN_BUF_OUTPUT = 1, KUN_BUF_TEMP = 2 };
 enum { KUN_STAGE_TS = 0, KUN_STAGE_RANK = 1, KUN_STAGE_SCALE = 2 };
 enum { KUN_LAYOUT_STS = 0, KUN_LAYOUT_TS = 1, KUN_LAYOUT_STREAM = 2 };
 enum { KUN_DTYPE_FLOAT = 0, KUN_DTYPE_DOUBLE = 1 };
+enum { KUN_KERNEL_PAR = 0, KUN_KERNEL_SEQ = 1 };
 
 typedef struct {
     const char *name;
@@ -22,14 +23,29 @@ typedef struct {
     int window;     /* rows of history consumers need */
 } KunBufferDesc;
 
+/* One kernel of a batch time-series stage. PAR kernels split the rows into
+ * chunks along grid y (each chunk recomputes `halo` rows of history before
+ * its first stored row); SEQ kernels walk every row in order. Grid z selects
+ * one of `ncomp` independent op groups. */
+typedef struct {
+    const void *kernel; /* host stub */
+    int kind;           /* KUN_KERNEL_* */
+    int ncomp;
+    int halo;
+    int lds_bytes;      /* static LDS per workgroup */
+    int spill_bytes;    /* HBM scratch bytes per (stock, chunk) column */
+    int io_values;      /* distinct HBM values read or written per (stock, row) */
+} KunKernelDesc;
+
 typedef struct {
     const char *name;
     int kind;             /* KUN_STAGE_* */
-    const void *kernel;   /* batch kernel host stub (TS stages) */
+    const KunKernelDesc *kerns; /* batch kernels of a TS stage, run in order */
+    int nkern;
+    int nmat;             /* [rows][stocks] arrays handed between the stage's kernels */
     const void *stream_kernel; /* one-tick kernel host stub (STREAM modules) */
     int block;            /* threads per workgroup */
-    int lds_bytes;        /* static LDS per workgroup */
-    int state_bytes;      /* per-stock HBM bytes: carried state (STREAM) or spilled rings (batch) */
+    int state_bytes;      /* per-stock HBM bytes of carried state (STREAM modules) */
     const int *in;
     int nin;
     const int *out;

@@ -39,24 +39,53 @@ struct Pool {
     ~Pool();
 };
 
+// Scratch of a batch run: per stage, the arrays handed between its kernels;
+// per kernel, its row chunking and HBM columns.
+struct BatchLayout {
+    struct Kern {
+        long chunks = 1, chunk_rows = 1;
+        size_t scratch = 0;
+    };
+    std::vector<size_t> stage_mat;
+    std::vector<std::vector<Kern>> kern;
+    size_t bytes = 0;
+};
+// How PAR kernels split their rows (defaults overridable by the environment
+// variables KUN_TARGET_WORKGROUPS, KUN_HALO_FACTOR, KUN_MIN_CHUNK_ROWS).
+struct ChunkPolicy {
+    long target_workgroups = 4096;
+    long halo_factor = 3;
+    long min_rows = 64;
+    static ChunkPolicy from_env();
+};
+BatchLayout batch_layout(const KunModuleDesc *m, size_t S, size_t L, const ChunkPolicy &pol);
+
+struct DeviceTemps {
+    std::vector<void *> bufs;
+    char *arena = nullptr;
+    BatchLayout lay;
+};
+
 // An executor: one device, one HIP stream, a buffer pool.
 struct Executor {
     int device = 0;
     hipStream_t stream = nullptr;
     Pool pool;
     std::mutex mu;
+    ChunkPolicy chunking;
     // temporaries of device-resident runs, per (module, stocks, rows)
-    std::map<std::tuple<const void *, size_t, size_t>, std::vector<void *>> temps;
+    std::map<std::tuple<const void *, size_t, size_t>, DeviceTemps> temps;
     // per-stage kernel timing (bench / diagnostics)
     bool profiling = false;
     struct Pending {
         const void *module;
-        int stage;
+        int stage, kern;
         hipEvent_t start, stop;
     };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> event_pool;
-    std::map<std::pair<const void *, int>, std::pair<double, long>> stage_ms;
+    // (module, stage, kernel of the stage) -> (milliseconds, launches)
+    std::map<std::tuple<const void *, int, int>, std::pair<double, long>> stage_ms;
     hipEvent_t take_event();
     void collect_profile();
     Executor();

@@ -4,9 +4,10 @@
 //
 // Replaces the reference's CPU runtime (cpp/Kun/Runtime.cpp, Executor.cpp,
 // Module.cpp, CApi.cpp): instead of a thread pool claiming 8-stock tasks, a
-// stage is one kernel launch over all stocks (TS stages: 64 stocks per
-// workgroup; cross-sectional stages: one workgroup per row), enqueued in a
-// topological order of the stage DAG on the executor's HIP stream.
+// stage is a short sequence of kernel launches over all stocks (TS stages:
+// 64 stocks per workgroup, rows split into chunks where the ops allow;
+// cross-sectional stages: one workgroup per row), enqueued in a topological
+// order of the stage DAG on the executor's HIP stream.
 #include <hip/hip_runtime.h>
 
 #include <dlfcn.h>
@@ -69,7 +70,7 @@ Pool::~Pool() {
     for (auto &kv : free_) hipFree(kv.second);
 }
 
-Executor::Executor() {
+Executor::Executor() : chunking(ChunkPolicy::from_env()) {
     HIPCHECK(hipGetDevice(&device));
     HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
 }
@@ -101,7 +102,7 @@ void Executor::collect_profile() {
         HIPCHECK(hipEventSynchronize(p.stop));
         float ms = 0;
         HIPCHECK(hipEventElapsedTime(&ms, p.start, p.stop));
-        auto &acc = stage_ms[{p.module, p.stage}];
+        auto &acc = stage_ms[std::make_tuple(p.module, p.stage, p.kern)];
         acc.first += ms;
         acc.second += 1;
         event_pool.push_back(p.start);
@@ -194,49 +195,113 @@ static kun_cs::View cs_view(const KunModuleDesc *m, int b, void *ptr, size_t T, 
     return v;
 }
 
-// Enqueue every stage of a batch module; dev[b] are device pointers,
-// scratch[s] the HBM scratch of stage s (or null).
-void enqueue_batch(const KunModuleDesc *m, const std::vector<void *> &dev, const std::vector<void *> &scratch,
+// ---------------------------------------------------------------------------
+// Launch geometry and HBM scratch of a batch run.
+
+static long env_long(const char *name, long dflt) {
+    const char *v = getenv(name);
+    return (v && *v) ? atol(v) : dflt;
+}
+
+ChunkPolicy ChunkPolicy::from_env() {
+    ChunkPolicy p;
+    p.target_workgroups = env_long("KUN_TARGET_WORKGROUPS", p.target_workgroups);
+    p.halo_factor = env_long("KUN_HALO_FACTOR", p.halo_factor);
+    p.min_rows = env_long("KUN_MIN_CHUNK_ROWS", p.min_rows);
+    return p;
+}
+
+// Row chunks of a PAR kernel: enough workgroups to fill the chip, each chunk
+// at least `halo_factor` times the halo it recomputes and `min_rows` long.
+static long pick_chunks(const KunKernelDesc &kd, size_t S, size_t L, const ChunkPolicy &pol) {
+    if (kd.kind != KUN_KERNEL_PAR || L == 0) return 1;
+    long nsb = ((long)S + 63) / 64;
+    long rows_min = std::max(std::max(1L, pol.min_rows), pol.halo_factor * (long)kd.halo);
+    long max_ch = std::max(1L, (long)L / rows_min);
+    long want = std::max(1L, (pol.target_workgroups + nsb * kd.ncomp - 1) / (nsb * kd.ncomp));
+    long nch = std::min(max_ch, want);
+    long chunk = ((long)L + nch - 1) / nch;
+    return ((long)L + chunk - 1) / chunk;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+BatchLayout batch_layout(const KunModuleDesc *m, size_t S, size_t L, const ChunkPolicy &pol) {
+    BatchLayout lay;
+    const size_t esz = elem_size(m);
+    lay.stage_mat.assign(m->nstage, 0);
+    lay.kern.resize(m->nstage);
+    size_t off = 0;
+    for (int i = 0; i < m->nstage; i++) {
+        const KunStageDesc &sd = m->stages[i];
+        if (sd.kind != KUN_STAGE_TS) continue;
+        if (sd.nmat) {
+            lay.stage_mat[i] = off;
+            off += align256((size_t)sd.nmat * L * S * esz);
+        }
+        for (int j = 0; j < sd.nkern; j++) {
+            BatchLayout::Kern k;
+            k.chunks = pick_chunks(sd.kerns[j], S, L, pol);
+            k.chunk_rows = L ? ((long)L + k.chunks - 1) / k.chunks : 1;
+            k.scratch = off;
+            off += align256((size_t)sd.kerns[j].spill_bytes * S * k.chunks);
+            lay.kern[i].push_back(k);
+        }
+    }
+    lay.bytes = off;
+    return lay;
+}
+
+// Enqueue every stage of a batch module; dev[b] are device pointers, arena
+// the scratch described by `lay`.
+void enqueue_batch(const KunModuleDesc *m, const std::vector<void *> &dev, const BatchLayout &lay, char *arena,
                    size_t S, size_t T, size_t start, size_t L, hipStream_t st, Executor *prof) {
     if (L == 0) return;
     for (int si : stage_order(m)) {
         const KunStageDesc &sd = m->stages[si];
-        Executor::Pending ev{m, si, nullptr, nullptr};
-        if (prof) {
-            ev.start = prof->take_event();
-            ev.stop = prof->take_event();
-            HIPCHECK(hipEventRecord(ev.start, st));
-        }
-        if (sd.kind == KUN_STAGE_TS) {
-            kun::StageArgs a;
-            memset(&a, 0, sizeof a);
-            int n = 0;
-            for (int j = 0; j < sd.nin; j++) a.bufs[n++] = dev[sd.in[j]];
-            for (int j = 0; j < sd.nout; j++) a.bufs[n++] = dev[sd.out[j]];
-            a.S = (long)S;
-            a.T = (long)T;
-            a.start = (long)start;
-            a.L = (long)L;
-            a.blk = m->blocking;
-            a.scratch = scratch[si];
-            void *args[] = {&a};
-            unsigned grid = (unsigned)((S + sd.block - 1) / sd.block);
-            HIPCHECK(hipLaunchKernel(sd.kernel, dim3(grid), dim3(sd.block), args, 0, st));
-        } else {
-            int ib = sd.in[0], ob = sd.out[0];
-            kun_cs::View vi = cs_view(m, ib, dev[ib], T, start, L);
-            kun_cs::View vo = cs_view(m, ob, dev[ob], T, start, L);
-            if (sd.kind == KUN_STAGE_RANK && (long)S > kun_cs::max_rank_stocks(m->dtype))
-                throw Error("Rank over more than " + std::to_string(kun_cs::max_rank_stocks(m->dtype)) +
-                            " stocks is not supported");
-            hipError_t e = sd.kind == KUN_STAGE_RANK
-                               ? kun_cs::launch_rank(m->dtype, vi, vo, (long)S, (long)L, m->blocking, st)
-                               : kun_cs::launch_scale(m->dtype, vi, vo, (long)S, (long)L, m->blocking, st);
-            if (e != hipSuccess) throw Error(std::string("cross-sectional kernel: ") + hipGetErrorString(e));
-        }
-        if (prof) {
-            HIPCHECK(hipEventRecord(ev.stop, st));
-            prof->pending.push_back(ev);
+        int nk = sd.kind == KUN_STAGE_TS ? sd.nkern : 1;
+        for (int kj = 0; kj < nk; kj++) {
+            Executor::Pending ev{m, si, kj, nullptr, nullptr};
+            if (prof) {
+                ev.start = prof->take_event();
+                ev.stop = prof->take_event();
+                HIPCHECK(hipEventRecord(ev.start, st));
+            }
+            if (sd.kind == KUN_STAGE_TS) {
+                const KunKernelDesc &kd = sd.kerns[kj];
+                const BatchLayout::Kern &kl = lay.kern[si][kj];
+                kun::StageArgs a;
+                memset(&a, 0, sizeof a);
+                int n = 0;
+                for (int j = 0; j < sd.nin; j++) a.bufs[n++] = dev[sd.in[j]];
+                for (int j = 0; j < sd.nout; j++) a.bufs[n++] = dev[sd.out[j]];
+                a.S = (long)S;
+                a.T = (long)T;
+                a.start = (long)start;
+                a.L = (long)L;
+                a.chunk = kl.chunk_rows;
+                a.blk = m->blocking;
+                a.scratch = kd.spill_bytes ? arena + kl.scratch : nullptr;
+                a.mat = sd.nmat ? arena + lay.stage_mat[si] : nullptr;
+                void *args[] = {&a};
+                dim3 grid((unsigned)((S + sd.block - 1) / sd.block), (unsigned)kl.chunks, (unsigned)kd.ncomp);
+                HIPCHECK(hipLaunchKernel(kd.kernel, grid, dim3(sd.block), args, 0, st));
+            } else {
+                int ib = sd.in[0], ob = sd.out[0];
+                kun_cs::View vi = cs_view(m, ib, dev[ib], T, start, L);
+                kun_cs::View vo = cs_view(m, ob, dev[ob], T, start, L);
+                if (sd.kind == KUN_STAGE_RANK && (long)S > kun_cs::max_rank_stocks(m->dtype))
+                    throw Error("Rank over more than " + std::to_string(kun_cs::max_rank_stocks(m->dtype)) +
+                                " stocks is not supported");
+                hipError_t e = sd.kind == KUN_STAGE_RANK
+                                   ? kun_cs::launch_rank(m->dtype, vi, vo, (long)S, (long)L, m->blocking, st)
+                                   : kun_cs::launch_scale(m->dtype, vi, vo, (long)S, (long)L, m->blocking, st);
+                if (e != hipSuccess) throw Error(std::string("cross-sectional kernel: ") + hipGetErrorString(e));
+            }
+            if (prof) {
+                HIPCHECK(hipEventRecord(ev.stop, st));
+                prof->pending.push_back(ev);
+            }
         }
     }
 }
@@ -277,21 +342,18 @@ void run_graph_host(Executor &ex, const KunModuleDesc *m, const NameMap &bufs, s
         if (d.kind == KUN_BUF_INPUT)
             HIPCHECK(hipMemcpyAsync(dev[b], bufs.at(d.name), bytes[b], hipMemcpyHostToDevice, ex.stream));
     }
-    std::vector<void *> scratch(m->nstage, nullptr);
-    struct ReleaseScratch {
+    BatchLayout lay = batch_layout(m, S, L, ex.chunking);
+    struct ReleaseArena {
         Executor &ex;
-        const KunModuleDesc *m;
-        std::vector<void *> &scratch;
-        size_t S;
-        ~ReleaseScratch() {
+        char *p = nullptr;
+        size_t bytes;
+        ~ReleaseArena() {
             hipStreamSynchronize(ex.stream);
-            for (int i = 0; i < m->nstage; i++) ex.pool.put(scratch[i], (size_t)m->stages[i].state_bytes * S);
+            ex.pool.put(p, bytes);
         }
-    } release_scratch{ex, m, scratch, S};
-    for (int i = 0; i < m->nstage; i++)
-        if (m->stages[i].kind == KUN_STAGE_TS && m->stages[i].state_bytes)
-            scratch[i] = ex.pool.get((size_t)m->stages[i].state_bytes * S);
-    enqueue_batch(m, dev, scratch, S, T, start, L, ex.stream, ex.profiling ? &ex : nullptr);
+    } arena{ex, nullptr, lay.bytes};
+    if (lay.bytes) arena.p = (char *)ex.pool.get(lay.bytes);
+    enqueue_batch(m, dev, lay, arena.p, S, T, start, L, ex.stream, ex.profiling ? &ex : nullptr);
     for (int b = 0; b < m->nbuf; b++) {
         const KunBufferDesc &d = m->bufs[b];
         if (d.kind == KUN_BUF_OUTPUT)
@@ -309,38 +371,41 @@ void run_graph_device(Executor &ex, const KunModuleDesc *m, const NameMap &bufs,
     if (!st) st = ex.stream;
     const size_t esz = elem_size(m);
     auto key = std::make_tuple((const void *)m, S, L);
-    auto &temps = ex.temps[key];
-    if (temps.empty()) {
-        // buffers first, then one scratch area per stage
-        temps.resize(m->nbuf + m->nstage, nullptr);
+    auto it = ex.temps.find(key);
+    if (it == ex.temps.end()) {
+        // temporaries first, then the scratch arena
+        DeviceTemps t;
+        t.lay = batch_layout(m, S, L, ex.chunking);
+        t.bufs.assign(m->nbuf, nullptr);
         for (int b = 0; b < m->nbuf; b++)
             if (m->bufs[b].kind == KUN_BUF_TEMP) {
                 size_t nb = buffer_elems(m, b, S, T, L) * esz;
-                HIPCHECK(hipMalloc(&temps[b], nb ? nb : 256));
+                HIPCHECK(hipMalloc(&t.bufs[b], nb ? nb : 256));
             }
-        for (int i = 0; i < m->nstage; i++)
-            if (m->stages[i].kind == KUN_STAGE_TS && m->stages[i].state_bytes)
-                HIPCHECK(hipMalloc(&temps[m->nbuf + i], (size_t)m->stages[i].state_bytes * S));
+        if (t.lay.bytes) HIPCHECK(hipMalloc((void **)&t.arena, t.lay.bytes));
+        it = ex.temps.emplace(key, std::move(t)).first;
     }
-    std::vector<void *> scratch(temps.begin() + m->nbuf, temps.end());
+    const DeviceTemps &temps = it->second;
     std::vector<void *> dev(m->nbuf, nullptr);
     for (int b = 0; b < m->nbuf; b++) {
         const KunBufferDesc &d = m->bufs[b];
         if (d.kind == KUN_BUF_TEMP) {
-            dev[b] = temps[b];
+            dev[b] = temps.bufs[b];
         } else {
-            auto it = bufs.find(d.name);
-            if (it == bufs.end()) throw Error(std::string("Buffer name not found: ") + d.name);
-            dev[b] = it->second;
+            auto f = bufs.find(d.name);
+            if (f == bufs.end()) throw Error(std::string("Buffer name not found: ") + d.name);
+            dev[b] = f->second;
         }
     }
-    enqueue_batch(m, dev, scratch, S, T, start, L, st, ex.profiling ? &ex : nullptr);
+    enqueue_batch(m, dev, temps.lay, temps.arena, S, T, start, L, st, ex.profiling ? &ex : nullptr);
 }
 
 void Executor::free_temps() {
-    for (auto &kv : temps)
-        for (void *p : kv.second)
+    for (auto &kv : temps) {
+        for (void *p : kv.second.bufs)
             if (p) hipFree(p);
+        if (kv.second.arena) hipFree(kv.second.arena);
+    }
     temps.clear();
 }
 

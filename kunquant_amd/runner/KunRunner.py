@@ -62,6 +62,11 @@ def _load():
     sig("kunModuleStageOutput", i, vp, i, i)
     sig("kunExecutorEnableProfiling", i, vp, i)
     sig("kunExecutorStageTime", i, vp, vp, i, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long))
+    sig("kunExecutorKernelTime", i, vp, vp, i, i, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long))
+    sig("kunExecutorSetChunking", i, vp, ctypes.c_long, ctypes.c_long, ctypes.c_long)
+    sig("kunModuleStageNumKernels", i, vp, i)
+    ip = ctypes.POINTER(ctypes.c_int)
+    sig("kunModuleKernelInfo", i, vp, i, i, ip, ip, ip, ip, ip)
     sig("kunCreateStreamEx", i, vp, vp, sz, vp, ctypes.POINTER(vp))
     sig("kunStreamSerializeStates", i, vp, i, vp, ctypes.POINTER(sz))
     sig("kunQueryBufferHandle", sz, vp, cp)
@@ -104,11 +109,24 @@ class Executor:
             raise RuntimeError(_err())
 
     def stage_time(self, mod: "Module", stage: int):
-        """(accumulated milliseconds, launches) of one stage kernel."""
+        """(accumulated milliseconds of all kernels of a stage, runs)."""
         ms, n = ctypes.c_double(), ctypes.c_long()
         if _load().kunExecutorStageTime(self._h, mod._h, stage, ctypes.byref(ms), ctypes.byref(n)):
             raise RuntimeError(_err())
         return ms.value, n.value
+
+    def kernel_time(self, mod: "Module", stage: int, kern: int):
+        """(accumulated milliseconds, launches) of one kernel of a stage."""
+        ms, n = ctypes.c_double(), ctypes.c_long()
+        if _load().kunExecutorKernelTime(self._h, mod._h, stage, kern, ctypes.byref(ms), ctypes.byref(n)):
+            raise RuntimeError(_err())
+        return ms.value, n.value
+
+    def set_chunking(self, target_workgroups: int = 4096, halo_factor: int = 3, min_rows: int = 64):
+        """How row-chunked stage kernels split the rows (results do not
+        depend on it)."""
+        if _load().kunExecutorSetChunking(self._h, target_workgroups, halo_factor, min_rows):
+            raise RuntimeError(_err())
 
 
 def createSingleThreadExecutor() -> Executor:
@@ -169,6 +187,21 @@ class Module:
             ins = [L.kunModuleStageInput(self._h, i, j) for j in range(L.kunModuleStageNumInputs(self._h, i))]
             outs = [L.kunModuleStageOutput(self._h, i, j) for j in range(L.kunModuleStageNumOutputs(self._h, i))]
             res.append((L.kunModuleStageName(self._h, i).decode(), L.kunModuleStageKind(self._h, i), ins, outs))
+        return res
+
+    def kernels(self, stage: int):
+        """[(kind, components, halo, lds bytes, HBM values per cell)] of the
+        kernels a time-series stage launches; kind 0 row-chunked, 1
+        sequential."""
+        L = _load()
+        if L.kunModuleStageKind(self._h, stage) != 0:
+            return []
+        res = []
+        for j in range(L.kunModuleStageNumKernels(self._h, stage)):
+            v = [ctypes.c_int() for _ in range(5)]
+            if L.kunModuleKernelInfo(self._h, stage, j, *[ctypes.byref(x) for x in v]):
+                raise RuntimeError(_err())
+            res.append(tuple(x.value for x in v))
         return res
 
 

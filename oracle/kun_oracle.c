@@ -328,8 +328,9 @@ int kun_oracle_run(void *prog, void **bufs, long S, long T, long start, long L, 
     for (int b = 0; b < pg->nbuf; b++) {
         const Buf *bi = &pg->bufs[b];
         if (bi->kind == 2) {
-            owned[b] = calloc((size_t)S * L, esz);
-            views[b].ptr = owned[b];
+            /* temporaries: [L][S], caller-provided when the caller wants them */
+            if (!bufs[b]) owned[b] = calloc((size_t)S * L, esz);
+            views[b].ptr = bufs[b] ? bufs[b] : owned[b];
             views[b].layout = 1;
             views[b].num_time = L;
         } else {

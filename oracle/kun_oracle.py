@@ -83,12 +83,19 @@ class OracleProgram:
         return [n for _, k, n in self.buffers if k == 1]
 
     def run(self, inputs: Dict[str, np.ndarray], num_stocks: int, total_time: int, cur_time: int,
-            length: int, nthreads: int = 1, outputs: Optional[Dict[str, np.ndarray]] = None):
+            length: int, nthreads: int = 1, outputs: Optional[Dict[str, np.ndarray]] = None,
+            temps: bool = False):
+        """Outputs by name; with temps=True also every temporary buffer
+        ([length][num_stocks])."""
         L = lib()
         ptrs = (ctypes.c_void_p * len(self.buffers))()
         keep = []
         result = {}
         for bid, kind, name in self.buffers:
+            if kind == 2 and temps:
+                arr = np.zeros((length, num_stocks), dtype=self.dtype)
+                result[name] = arr
+                ptrs[bid] = arr.ctypes.data
             if kind == 0:
                 arr = np.ascontiguousarray(inputs[name], dtype=self.dtype)
                 keep.append(arr)

@@ -4,7 +4,8 @@ import pytest
 from kunquant_amd.Driver import KunCompilerConfig, compileit
 from kunquant_amd.Op import Builder, Input, Output, Rank
 from kunquant_amd.Stage import Function
-from kunquant_amd.codegen.hip import LDS_LIMIT, StageEmitter, emit_module
+from kunquant_amd.codegen.hip import LDS_LIMIT, LevelKernel, StageEmitter, emit_module, stage_plan
+from kunquant_amd.codegen.plan import PAR, REC_OPS, SEQ, plan_stage
 from kunquant_amd.ops import DiffWithWeightedSum, ExpMovingAvg, WindowedSum
 from kunquant_amd.workloads import catalog
 
@@ -15,9 +16,14 @@ def test_emit_catalog(name):
     src = emit_module(m)
     assert f"kun_module_{name}" in src
     for st in m.stages:
-        if st.kind == "ts":
-            em = StageEmitter(m, st, "k")
-            assert em.lds_bytes <= LDS_LIMIT
+        if st.kind != "ts":
+            continue
+        if m.stream_mode:
+            StageEmitter(m, st, "k")
+            continue
+        plan = stage_plan(m, st)
+        for j, level in enumerate(plan.levels):
+            assert LevelKernel(m, st, plan, level, f"k{j}").lds_bytes <= LDS_LIMIT
 
 
 def test_driver_option_errors():
@@ -55,3 +61,74 @@ def test_ema_init_rejected_in_stream():
     m = compileit(Function(b.ops), "m", input_layout="STREAM", output_layout="STREAM")
     with pytest.raises(RuntimeError, match="EMA with init_val is not supported in stream mode"):
         emit_module(m)
+
+
+def _batch_stage_plans():
+    for name in sorted(catalog.CATALOG):
+        m = catalog.module_ir(name)
+        if m.stream_mode:
+            continue
+        for st in m.stages:
+            if st.kind == "ts":
+                yield name, st, plan_stage(st.func.ops)
+
+
+def test_plan_invariants():
+    """Every op lands in one component; recurrences only on sequential
+    levels; values crossing levels are materialised by a lower level."""
+    from kunquant_amd.Op import ConstantOp, ForeachBackWindow, Input
+    for name, st, plan in _batch_stage_plans():
+        ops = st.func.ops
+        seen = {}
+        for li, level in enumerate(plan.levels):
+            for ci, comp in enumerate(level.comps):
+                members = set(comp.ops)
+                for op in comp.ops:
+                    if isinstance(op, (Input, ConstantOp)):
+                        continue
+                    assert op not in seen, (name, st.idx)
+                    seen[op] = li
+                    cls = op.__class__.__name__
+                    if cls in REC_OPS:
+                        assert level.kind == SEQ
+                    if isinstance(op, ForeachBackWindow) and plan.fallback is None:
+                        assert level.kind == PAR
+                for op in comp.ops:
+                    for inp in op.inputs:
+                        if inp in members or isinstance(inp, (Input, ConstantOp)):
+                            continue
+                        assert inp in comp.ext and inp in plan.mat, (name, st.idx, op, inp)
+                        assert seen[inp] < li
+        computed = [op for op in ops if not isinstance(op, (Input, ConstantOp))]
+        assert set(seen) == set(computed)
+        if plan.fallback is not None:
+            assert plan.fallback == "accumulator", (name, st.idx, plan.fallback)
+
+
+def test_plan_halo():
+    """Halo of a PAR component = longest chain of in-kernel window offsets:
+    Delay(WindowedMax(WindowedMin(x, 5), 7), 3) recomputes 6 + 3 rows; the
+    window over the input itself is rebuilt from HBM and costs none."""
+    from kunquant_amd.ops import BackRef, WindowedMax, WindowedMin
+    b = Builder()
+    with b:
+        Output(BackRef(WindowedMax(WindowedMin(Input("x"), 5), 7), 3), "o")
+    m = compileit(Function(b.ops), "halo", input_layout="TS", output_layout="TS")
+    (st,) = [s for s in m.stages if s.kind == "ts"]
+    plan = stage_plan(m, st)
+    assert [lv.kind for lv in plan.levels] == [PAR]
+    assert LevelKernel(m, st, plan, plan.levels[0], "k").halo == 9
+
+
+def test_plan_levels_of_running_sum():
+    """ts_mean feeding a window loop: PAR (nothing) -> SEQ running sum ->
+    PAR loop over the materialised mean."""
+    from kunquant_amd.ops import WindowedAvg, WindowedMax
+    b = Builder()
+    with b:
+        Output(WindowedMax(WindowedAvg(Input("x"), 10), 5), "o")
+    m = compileit(Function(b.ops), "lv", input_layout="TS", output_layout="TS")
+    (st,) = [s for s in m.stages if s.kind == "ts"]
+    plan = stage_plan(m, st)
+    assert [lv.kind for lv in plan.levels] == [SEQ, PAR]
+    assert len(plan.mat) == 1

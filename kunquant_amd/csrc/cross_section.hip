@@ -4,15 +4,16 @@
 // sorted; a value's rank is the average 1-based position of its tie group
 // [lo, hi): sum = (lo + hi + 1) * (hi - lo) / 2 in integers, then
 // T(sum) / T(hi - lo) / T(n). -0 and +0 tie (std::sort / equal_range use
-// operator<). On the device a row is sorted with an LDS block radix sort of
-// order-preserving integer keys (-0 canonicalised to +0, NaN to the largest
-// key) and every element finds its group by two binary searches.
+// operator<). On the device a row's (key, stock) pairs are sorted with an
+// LDS block radix sort of order-preserving integer keys (-0 canonicalised to
+// +0, NaN to the largest key) and tie groups are read off the sorted order.
 //
 // Scale (restates cpp/Kun/Scale.hpp:13-46): sum of |x| over non-NaN values
 // accumulated sequentially in stock order (bit-exact order), then
 // x / sum, or NaN when x == 0 and sum == 0.
 #include <hip/hip_runtime.h>
-#include <hipcub/block/block_radix_sort.hpp>
+#include <rocprim/block/block_radix_sort.hpp>
+#include <rocprim/block/block_scan.hpp>
 #include <stdint.h>
 
 #include "kun_cs.hpp"
@@ -39,15 +40,26 @@ __device__ __forceinline__ size_t cs_at(int layout, long row, long s, long S, lo
     return ((size_t)(s / blk) * rows + row) * blk + s % blk;
 }
 
+// One workgroup per row. (key, stock) pairs are radix sorted; in the sorted
+// order the tie group [a, b) of position p is found without searching:
+// a = the last group start at or before p (a block max-scan of group
+// starts), b = the end recorded by the group's last element. Ranks go to LDS
+// at their stock index and leave in one coalesced pass.
 template <typename T, int BLOCK, int ITEMS>
 __global__ __launch_bounds__(BLOCK) void rank_rows(kun_cs::View in, kun_cs::View out, long S, int blk) {
     using K = typename KeyOf<T>::K;
-    using Sort = hipcub::BlockRadixSort<K, BLOCK, ITEMS>;
+    constexpr int N = BLOCK * ITEMS;
+    using Sort = rocprim::block_radix_sort<K, BLOCK, ITEMS, int>;
+    using Scan = rocprim::block_scan<int, BLOCK>;
+    static_assert(sizeof(T) <= sizeof(K) && sizeof(int) <= sizeof(K), "LDS aliasing");
     __shared__ union {
-        typename Sort::TempStorage sort;
-        K sorted[BLOCK * ITEMS];
+        typename Sort::storage_type sort;
+        K skey[N];  // sorted keys; then group ends (int); then ranks by stock (T)
     } sh;
+    __shared__ typename Scan::storage_type scan;
     __shared__ int nvalid;
+    int *endof = (int *)sh.skey;
+    T *res = (T *)sh.skey;
     const long t = blockIdx.x;
     const T *src = (const T *)in.ptr;
     T *dst = (T *)out.ptr;
@@ -55,6 +67,7 @@ __global__ __launch_bounds__(BLOCK) void rank_rows(kun_cs::View in, kun_cs::View
     if (threadIdx.x == 0) nvalid = 0;
     __syncthreads();
     K keys[ITEMS];
+    int idx[ITEMS];
     int cnt = 0;
 #pragma unroll
     for (int j = 0; j < ITEMS; j++) {
@@ -66,73 +79,115 @@ __global__ __launch_bounds__(BLOCK) void rank_rows(kun_cs::View in, kun_cs::View
             cnt += (x == x) ? 1 : 0;
         }
         keys[j] = k;
+        idx[j] = (int)s;
     }
     if (cnt) atomicAdd(&nvalid, cnt);
-    Sort(sh.sort).Sort(keys);
+    Sort().sort(keys, idx, sh.sort);
     __syncthreads();
+    const int p0 = threadIdx.x * ITEMS;  // blocked: this thread holds sorted positions p0 .. p0+ITEMS-1
 #pragma unroll
-    for (int j = 0; j < ITEMS; j++) sh.sorted[threadIdx.x * ITEMS + j] = keys[j];
+    for (int j = 0; j < ITEMS; j++) sh.skey[p0 + j] = keys[j];
     __syncthreads();
     const int n = nvalid;
-    for (long s = threadIdx.x; s < S; s += BLOCK) {
-        T x = src[cs_at(in.layout, rin, s, S, in.rows, blk)];
-        T r;
-        if (x != x) {
-            r = kun::qnan<T>();
-        } else {
-            K k = rank_key(x);
-            int lo = 0, hi = n;
-            while (lo < hi) {
-                int mid = (lo + hi) >> 1;
-                if (sh.sorted[mid] < k) lo = mid + 1; else hi = mid;
-            }
-            int a = lo;
-            hi = n;
-            while (lo < hi) {
-                int mid = (lo + hi) >> 1;
-                if (k < sh.sorted[mid]) hi = mid; else lo = mid + 1;
-            }
-            int b = lo;
-            int64_t sum = ((int64_t)a + b + 1) * (int64_t)(b - a) / 2;
-            r = (T)sum / (T)(b - a) / (T)n;
-        }
-        dst[cs_at(out.layout, t + out.row0, s, S, out.rows, blk)] = r;
+    int start[ITEMS];
+    bool last[ITEMS];
+    int run = -1;
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        const int p = p0 + j;
+        if (p == 0 || sh.skey[p - 1] != keys[j]) run = p;
+        start[j] = run;
+        last[j] = p < n && (p == n - 1 || sh.skey[p + 1] != keys[j]);
     }
+    int prefix;
+    Scan().exclusive_scan(run, prefix, -1, scan, rocprim::maximum<int>());
+    __syncthreads();  // every read of the sorted keys is done; reuse their LDS
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        start[j] = start[j] > prefix ? start[j] : prefix;
+        if (last[j]) endof[start[j]] = p0 + j + 1;
+    }
+    __syncthreads();
+    T r[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        const int p = p0 + j;
+        if (p < n) {
+            const int a = start[j], b = endof[a];
+            const int64_t sum = ((int64_t)a + b + 1) * (int64_t)(b - a) / 2;
+            r[j] = (T)sum / (T)(b - a) / (T)n;
+        } else {
+            r[j] = kun::qnan<T>();
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++)
+        if (idx[j] < S) res[idx[j]] = r[j];
+    __syncthreads();
+    for (long s = threadIdx.x; s < S; s += BLOCK) dst[cs_at(out.layout, t + out.row0, s, S, out.rows, blk)] = res[s];
 }
 
+// SCALE_ROWS rows per workgroup; the sum of each row is one sequential chain
+// (lane r of the first wave owns row r), fed from LDS tiles that the whole
+// workgroup loads one tile ahead. Adding +0 for NaN cells leaves a sum of
+// non-negative terms bit-identical, so the chain is one add per stock.
 constexpr int SCALE_BLOCK = 256;
-constexpr int SCALE_CHUNK = 2048;
+constexpr int SCALE_ROWS = 8;
+constexpr int SCALE_TILE = 1024;
 
 template <typename T>
-__global__ __launch_bounds__(SCALE_BLOCK) void scale_rows(kun_cs::View in, kun_cs::View out, long S, int blk) {
-    __shared__ T chunk[SCALE_CHUNK];
-    __shared__ T total;
-    const long t = blockIdx.x;
+__global__ __launch_bounds__(SCALE_BLOCK) void scale_rows(kun_cs::View in, kun_cs::View out, long S, long rows,
+                                                           int blk) {
+    constexpr int PER = SCALE_ROWS * SCALE_TILE / SCALE_BLOCK;
+    __shared__ T tile[SCALE_ROWS * (SCALE_TILE + 1)];
+    __shared__ T total[SCALE_ROWS];
+    const long r0 = (long)blockIdx.x * SCALE_ROWS;
+    const int nr = (int)((rows - r0) < SCALE_ROWS ? (rows - r0) : SCALE_ROWS);
     const T *src = (const T *)in.ptr;
     T *dst = (T *)out.ptr;
-    const long rin = t + in.row0;
+    T reg[PER];
+    auto load = [&](long c0) {
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int e = threadIdx.x + q * SCALE_BLOCK;
+            const int r = e / SCALE_TILE, c = e % SCALE_TILE;
+            const long s = c0 + c;
+            reg[q] = (r < nr && s < S) ? src[cs_at(in.layout, r0 + r + in.row0, s, S, in.rows, blk)] : T(0);
+        }
+    };
+    load(0);
     T sum = T(0);
-    for (long c0 = 0; c0 < S; c0 += SCALE_CHUNK) {
-        int n = (int)((S - c0) < SCALE_CHUNK ? (S - c0) : SCALE_CHUNK);
-        for (int k = threadIdx.x; k < n; k += SCALE_BLOCK) chunk[k] = src[cs_at(in.layout, rin, c0 + k, S, in.rows, blk)];
+    for (long c0 = 0; c0 < S; c0 += SCALE_TILE) {
         __syncthreads();
-        if (threadIdx.x == 0) {
-            // sequential in stock order, as the reference's scalar loop
-#pragma unroll 8
-            for (int k = 0; k < n; k++) {
-                T x = chunk[k];
-                if (x == x) sum = sum + kun::vabs(x);
-            }
+#pragma unroll
+        for (int q = 0; q < PER; q++) {
+            const int e = threadIdx.x + q * SCALE_BLOCK;
+            tile[(e / SCALE_TILE) * (SCALE_TILE + 1) + e % SCALE_TILE] = reg[q];
         }
         __syncthreads();
+        if (c0 + SCALE_TILE < S) load(c0 + SCALE_TILE);
+        if ((int)threadIdx.x < nr) {
+            const int n = (int)((S - c0) < SCALE_TILE ? (S - c0) : SCALE_TILE);
+            const T *row = tile + threadIdx.x * (SCALE_TILE + 1);
+            // in stock order, as the reference's scalar loop
+#pragma unroll 16
+            for (int k = 0; k < n; k++) {
+                const T x = row[k];
+                sum = sum + ((x == x) ? kun::vabs(x) : T(0));
+            }
+        }
     }
-    if (threadIdx.x == 0) total = sum;
+    if ((int)threadIdx.x < nr) total[threadIdx.x] = sum;
     __syncthreads();
-    sum = total;
-    for (long s = threadIdx.x; s < S; s += SCALE_BLOCK) {
-        T x = src[cs_at(in.layout, rin, s, S, in.rows, blk)];
-        dst[cs_at(out.layout, t + out.row0, s, S, out.rows, blk)] =
-            (x == T(0) && sum == T(0)) ? kun::qnan<T>() : x / sum;
+    for (int r = 0; r < nr; r++) {
+        const T tot = total[r];
+        const long rr = r0 + r;
+        for (long s = threadIdx.x; s < S; s += SCALE_BLOCK) {
+            const T x = src[cs_at(in.layout, rr + in.row0, s, S, in.rows, blk)];
+            dst[cs_at(out.layout, rr + out.row0, s, S, out.rows, blk)] =
+                (x == T(0) && tot == T(0)) ? kun::qnan<T>() : x / tot;
+        }
     }
 }
 
@@ -169,10 +224,11 @@ hipError_t launch_rank(int dtype, const View &in, const View &out, long S, long 
 
 hipError_t launch_scale(int dtype, const View &in, const View &out, long S, long rows, int blk, hipStream_t st) {
     if (rows <= 0) return hipSuccess;
+    const unsigned grid = (unsigned)((rows + SCALE_ROWS - 1) / SCALE_ROWS);
     if (dtype)
-        hipLaunchKernelGGL(scale_rows<double>, dim3((unsigned)rows), dim3(SCALE_BLOCK), 0, st, in, out, S, blk);
+        hipLaunchKernelGGL(scale_rows<double>, dim3(grid), dim3(SCALE_BLOCK), 0, st, in, out, S, rows, blk);
     else
-        hipLaunchKernelGGL(scale_rows<float>, dim3((unsigned)rows), dim3(SCALE_BLOCK), 0, st, in, out, S, blk);
+        hipLaunchKernelGGL(scale_rows<float>, dim3(grid), dim3(SCALE_BLOCK), 0, st, in, out, S, rows, blk);
     return hipGetLastError();
 }
 

@@ -66,10 +66,16 @@ struct DeviceTemps {
     BatchLayout lay;
 };
 
-// An executor: one device, one HIP stream, a buffer pool.
+// An executor: one device, one HIP stream that calls are ordered on, a few
+// lanes (streams) that independent stages of a batch run spread over, a
+// buffer pool.
 struct Executor {
     int device = 0;
     hipStream_t stream = nullptr;
+    std::vector<hipStream_t> lanes;
+    std::vector<hipEvent_t> lane_done, stage_done;
+    hipEvent_t fork = nullptr;
+    void ensure_stage_events(int n);
     Pool pool;
     std::mutex mu;
     ChunkPolicy chunking;

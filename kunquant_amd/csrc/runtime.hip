@@ -33,6 +33,11 @@ namespace kunrt {
 
 thread_local std::string g_last_error;
 
+static long env_long(const char *name, long dflt) {
+    const char *v = getenv(name);
+    return (v && *v) ? atol(v) : dflt;
+}
+
 void set_error(const std::string &msg) { g_last_error = msg; }
 const char *last_error() { return g_last_error.c_str(); }
 
@@ -73,10 +78,35 @@ Pool::~Pool() {
 Executor::Executor() : chunking(ChunkPolicy::from_env()) {
     HIPCHECK(hipGetDevice(&device));
     HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    // lanes for independent stages (default priority, like every stream here)
+    long nl = env_long("KUN_STAGE_LANES", 4);
+    nl = nl < 0 ? 0 : (nl > 8 ? 8 : nl);
+    HIPCHECK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    for (long l = 0; l < nl; l++) {
+        hipStream_t s;
+        hipEvent_t e;
+        HIPCHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        lanes.push_back(s);
+        lane_done.push_back(e);
+    }
+}
+
+void Executor::ensure_stage_events(int n) {
+    while ((int)stage_done.size() < n) {
+        hipEvent_t e;
+        HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        stage_done.push_back(e);
+    }
 }
 
 Executor::~Executor() {
     hipStreamSynchronize(stream);
+    for (auto l : lanes) hipStreamSynchronize(l);
+    for (auto l : lanes) hipStreamDestroy(l);
+    for (auto e : lane_done) hipEventDestroy(e);
+    for (auto e : stage_done) hipEventDestroy(e);
+    if (fork) hipEventDestroy(fork);
     for (auto &p : pending) {
         hipEventDestroy(p.start);
         hipEventDestroy(p.stop);
@@ -198,11 +228,6 @@ static kun_cs::View cs_view(const KunModuleDesc *m, int b, void *ptr, size_t T, 
 // ---------------------------------------------------------------------------
 // Launch geometry and HBM scratch of a batch run.
 
-static long env_long(const char *name, long dflt) {
-    const char *v = getenv(name);
-    return (v && *v) ? atol(v) : dflt;
-}
-
 ChunkPolicy ChunkPolicy::from_env() {
     ChunkPolicy p;
     p.target_workgroups = env_long("KUN_TARGET_WORKGROUPS", p.target_workgroups);
@@ -252,57 +277,92 @@ BatchLayout batch_layout(const KunModuleDesc *m, size_t S, size_t L, const Chunk
     return lay;
 }
 
-// Enqueue every stage of a batch module; dev[b] are device pointers, arena
-// the scratch described by `lay`.
-void enqueue_batch(const KunModuleDesc *m, const std::vector<void *> &dev, const BatchLayout &lay, char *arena,
-                   size_t S, size_t T, size_t start, size_t L, hipStream_t st, Executor *prof) {
-    if (L == 0) return;
-    for (int si : stage_order(m)) {
-        const KunStageDesc &sd = m->stages[si];
-        int nk = sd.kind == KUN_STAGE_TS ? sd.nkern : 1;
-        for (int kj = 0; kj < nk; kj++) {
-            Executor::Pending ev{m, si, kj, nullptr, nullptr};
-            if (prof) {
-                ev.start = prof->take_event();
-                ev.stop = prof->take_event();
-                HIPCHECK(hipEventRecord(ev.start, st));
-            }
-            if (sd.kind == KUN_STAGE_TS) {
-                const KunKernelDesc &kd = sd.kerns[kj];
-                const BatchLayout::Kern &kl = lay.kern[si][kj];
-                kun::StageArgs a;
-                memset(&a, 0, sizeof a);
-                int n = 0;
-                for (int j = 0; j < sd.nin; j++) a.bufs[n++] = dev[sd.in[j]];
-                for (int j = 0; j < sd.nout; j++) a.bufs[n++] = dev[sd.out[j]];
-                a.S = (long)S;
-                a.T = (long)T;
-                a.start = (long)start;
-                a.L = (long)L;
-                a.chunk = kl.chunk_rows;
-                a.blk = m->blocking;
-                a.scratch = kd.spill_bytes ? arena + kl.scratch : nullptr;
-                a.mat = sd.nmat ? arena + lay.stage_mat[si] : nullptr;
-                void *args[] = {&a};
-                dim3 grid((unsigned)((S + sd.block - 1) / sd.block), (unsigned)kl.chunks, (unsigned)kd.ncomp);
-                HIPCHECK(hipLaunchKernel(kd.kernel, grid, dim3(sd.block), args, 0, st));
-            } else {
-                int ib = sd.in[0], ob = sd.out[0];
-                kun_cs::View vi = cs_view(m, ib, dev[ib], T, start, L);
-                kun_cs::View vo = cs_view(m, ob, dev[ob], T, start, L);
-                if (sd.kind == KUN_STAGE_RANK && (long)S > kun_cs::max_rank_stocks(m->dtype))
-                    throw Error("Rank over more than " + std::to_string(kun_cs::max_rank_stocks(m->dtype)) +
-                                " stocks is not supported");
-                hipError_t e = sd.kind == KUN_STAGE_RANK
-                                   ? kun_cs::launch_rank(m->dtype, vi, vo, (long)S, (long)L, m->blocking, st)
-                                   : kun_cs::launch_scale(m->dtype, vi, vo, (long)S, (long)L, m->blocking, st);
-                if (e != hipSuccess) throw Error(std::string("cross-sectional kernel: ") + hipGetErrorString(e));
-            }
-            if (prof) {
-                HIPCHECK(hipEventRecord(ev.stop, st));
-                prof->pending.push_back(ev);
-            }
+// Enqueue one stage (all its kernels) on stream `st`.
+static void enqueue_stage(const KunModuleDesc *m, int si, const std::vector<void *> &dev, const BatchLayout &lay,
+                          char *arena, size_t S, size_t T, size_t start, size_t L, hipStream_t st, Executor *prof) {
+    const KunStageDesc &sd = m->stages[si];
+    int nk = sd.kind == KUN_STAGE_TS ? sd.nkern : 1;
+    for (int kj = 0; kj < nk; kj++) {
+        Executor::Pending ev{m, si, kj, nullptr, nullptr};
+        if (prof) {
+            ev.start = prof->take_event();
+            ev.stop = prof->take_event();
+            HIPCHECK(hipEventRecord(ev.start, st));
         }
+        if (sd.kind == KUN_STAGE_TS) {
+            const KunKernelDesc &kd = sd.kerns[kj];
+            const BatchLayout::Kern &kl = lay.kern[si][kj];
+            kun::StageArgs a;
+            memset(&a, 0, sizeof a);
+            int n = 0;
+            for (int j = 0; j < sd.nin; j++) a.bufs[n++] = dev[sd.in[j]];
+            for (int j = 0; j < sd.nout; j++) a.bufs[n++] = dev[sd.out[j]];
+            a.S = (long)S;
+            a.T = (long)T;
+            a.start = (long)start;
+            a.L = (long)L;
+            a.chunk = kl.chunk_rows;
+            a.blk = m->blocking;
+            a.scratch = kd.spill_bytes ? arena + kl.scratch : nullptr;
+            a.mat = sd.nmat ? arena + lay.stage_mat[si] : nullptr;
+            void *args[] = {&a};
+            dim3 grid((unsigned)((S + sd.block - 1) / sd.block), (unsigned)kl.chunks, (unsigned)kd.ncomp);
+            HIPCHECK(hipLaunchKernel(kd.kernel, grid, dim3(sd.block), args, 0, st));
+        } else {
+            int ib = sd.in[0], ob = sd.out[0];
+            kun_cs::View vi = cs_view(m, ib, dev[ib], T, start, L);
+            kun_cs::View vo = cs_view(m, ob, dev[ob], T, start, L);
+            if (sd.kind == KUN_STAGE_RANK && (long)S > kun_cs::max_rank_stocks(m->dtype))
+                throw Error("Rank over more than " + std::to_string(kun_cs::max_rank_stocks(m->dtype)) +
+                            " stocks is not supported");
+            hipError_t e = sd.kind == KUN_STAGE_RANK
+                               ? kun_cs::launch_rank(m->dtype, vi, vo, (long)S, (long)L, m->blocking, st)
+                               : kun_cs::launch_scale(m->dtype, vi, vo, (long)S, (long)L, m->blocking, st);
+            if (e != hipSuccess) throw Error(std::string("cross-sectional kernel: ") + hipGetErrorString(e));
+        }
+        if (prof) {
+            HIPCHECK(hipEventRecord(ev.stop, st));
+            prof->pending.push_back(ev);
+        }
+    }
+}
+
+// Enqueue every stage of a batch module; dev[b] are device pointers, arena
+// the scratch described by `lay`. Stages run on the executor's lanes (HIP
+// streams) in a topological order, round robin; a stage waits for the
+// stages it reads from through events, and `st` waits for every lane at the
+// end (and every lane for `st` at the start).
+void enqueue_batch(Executor &ex, const KunModuleDesc *m, const std::vector<void *> &dev, const BatchLayout &lay,
+                   char *arena, size_t S, size_t T, size_t start, size_t L, hipStream_t st) {
+    if (L == 0) return;
+    Executor *prof = ex.profiling ? &ex : nullptr;
+    const std::vector<int> &order = stage_order(m);
+    const int nl = (int)ex.lanes.size();
+    if (nl == 0 || m->nstage == 1) {
+        for (int si : order) enqueue_stage(m, si, dev, lay, arena, S, T, start, L, st, prof);
+        return;
+    }
+    ex.ensure_stage_events(m->nstage);
+    HIPCHECK(hipEventRecord(ex.fork, st));
+    for (int l = 0; l < nl; l++) HIPCHECK(hipStreamWaitEvent(ex.lanes[l], ex.fork, 0));
+    std::vector<int> lane_of(m->nstage, 0);
+    int next = 0;
+    for (int si : order) {
+        const KunStageDesc &sd = m->stages[si];
+        const int l = next;
+        next = (next + 1) % nl;
+        lane_of[si] = l;
+        hipStream_t ls = ex.lanes[l];
+        for (int d = 0; d < sd.ndeps; d++) {
+            int dep = sd.deps[d];
+            if (lane_of[dep] != l) HIPCHECK(hipStreamWaitEvent(ls, ex.stage_done[dep], 0));
+        }
+        enqueue_stage(m, si, dev, lay, arena, S, T, start, L, ls, prof);
+        HIPCHECK(hipEventRecord(ex.stage_done[si], ls));
+    }
+    for (int l = 0; l < nl; l++) {
+        HIPCHECK(hipEventRecord(ex.lane_done[l], ex.lanes[l]));
+        HIPCHECK(hipStreamWaitEvent(st, ex.lane_done[l], 0));
     }
 }
 
@@ -353,7 +413,7 @@ void run_graph_host(Executor &ex, const KunModuleDesc *m, const NameMap &bufs, s
         }
     } arena{ex, nullptr, lay.bytes};
     if (lay.bytes) arena.p = (char *)ex.pool.get(lay.bytes);
-    enqueue_batch(m, dev, lay, arena.p, S, T, start, L, ex.stream, ex.profiling ? &ex : nullptr);
+    enqueue_batch(ex, m, dev, lay, arena.p, S, T, start, L, ex.stream);
     for (int b = 0; b < m->nbuf; b++) {
         const KunBufferDesc &d = m->bufs[b];
         if (d.kind == KUN_BUF_OUTPUT)
@@ -397,7 +457,7 @@ void run_graph_device(Executor &ex, const KunModuleDesc *m, const NameMap &bufs,
             dev[b] = f->second;
         }
     }
-    enqueue_batch(m, dev, temps.lay, temps.arena, S, T, start, L, st, ex.profiling ? &ex : nullptr);
+    enqueue_batch(ex, m, dev, temps.lay, temps.arena, S, T, start, L, st);
 }
 
 void Executor::free_temps() {

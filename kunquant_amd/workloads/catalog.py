@@ -6,7 +6,7 @@ import os
 from typing import Callable, Dict, Tuple
 
 from .. import PACKAGE_DIR
-from .graphs import aligned_graph, ops_graph, random_graph, simple_graph, stream_graph
+from .graphs import aligned_graph, cross_section_graph, ops_graph, random_graph, simple_graph, stream_graph
 
 MODULE_DIR = os.path.join(PACKAGE_DIR, "lib", "modules")
 
@@ -25,6 +25,9 @@ CATALOG: Dict[str, Tuple[Callable, dict]] = {
     "streamgraph_batch_f64": (stream_graph, dict(dtype="double", input_layout="TS", output_layout="TS",
                                                   options={"no_fast_stat": True})),
     "aligned_f32": (aligned_graph, dict(input_layout="TS", output_layout="TS", allow_unaligned=False)),
+    "cs_f32": (cross_section_graph, dict(input_layout="TS", output_layout="TS")),
+    "cs_f64": (cross_section_graph, dict(dtype="double", input_layout="TS", output_layout="TS",
+                                        options={"no_fast_stat": "no_warn"})),
     "stream_f64": (stream_graph, dict(dtype="double", input_layout="STREAM", output_layout="STREAM")),
     "stream_f32": (stream_graph, dict(input_layout="STREAM", output_layout="STREAM",
                                       options={"opt_reduce": False})),

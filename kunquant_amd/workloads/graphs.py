@@ -198,3 +198,17 @@ def aligned_graph(K=None):
     with b:
         K.Output(K.Rank(K.Input("a")) * 2, "ou2")
     return K.Function(b.ops)
+
+
+def cross_section_graph(K=None):
+    """Rank and Scale straight off the inputs (cross-sectional kernel checks)."""
+    K = K or native()
+    b = K.Builder()
+    with b:
+        a = K.Input("a")
+        c = K.Input("b")
+        K.Output(K.Rank(a), "rank_a")
+        K.Output(K.Scale(a), "scale_a")
+        K.Output(K.Rank(c), "rank_b")
+        K.Output(K.Scale(c), "scale_b")
+    return K.Function(b.ops)

@@ -141,3 +141,29 @@ def test_full_size_config2(executor):
         assert np.array_equal(got[k].view(np.uint32), again[k].view(np.uint32)), k
     exp = _oracle("cfg2_f32").run(d, S, T, 0, T, nthreads=NTHREADS)
     assert_same(got, exp, "config-2 full size")
+
+
+def _cs_rows(S, dt, seed):
+    rng = np.random.default_rng(seed)
+    rows = [np.full(S, np.nan), np.zeros(S), np.round(rng.random(S) * 3), rng.choice([-0.0, 0.0, 1.0], S),
+            rng.choice([-np.inf, np.inf, 2.0, np.nan], S), np.where(np.arange(S) == S // 2, 5.0, np.nan)]
+    for _ in range(6):
+        x = rng.standard_normal(S)
+        x[rng.random(S) < 0.1] = np.nan
+        rows.append(x)
+    return np.ascontiguousarray(np.stack(rows).astype(dt))
+
+
+@pytest.mark.parametrize("module", ["cs_f32", "cs_f64"])
+@pytest.mark.parametrize("S", [1, 5, 64, 257, 1000, 8192, 16384])
+def test_cross_section_edges(executor, module, S):
+    """Rank / Scale rows: all NaN, all zero, heavy ties, signed zeros,
+    infinities, a single valid value, random rows with NaN cells."""
+    _, mod = catalog.load(module)
+    a = _cs_rows(S, mod.dtype, S)
+    b = _cs_rows(S, mod.dtype, S + 1)[::-1].copy()
+    ins = {"a": a, "b": b}
+    T = a.shape[0]
+    got = kr.runGraph(executor, mod, ins, 0, T)
+    exp = _oracle(module).run(ins, S, T, 0, T, nthreads=NTHREADS)
+    assert_same(got, exp, f"{module} S={S}")

@@ -2,9 +2,10 @@
 the device templates in csrc/kun_device.hpp.
 
 Batch time-series stages follow the kernel plan of codegen/plan.py: each
-level becomes one kernel; its independent components are selected by
-blockIdx.z, blockIdx.x picks 64 stocks (one lane per stock, TS rows
-coalesced) and, for PAR levels, blockIdx.y picks a chunk of rows.
+level becomes one kernel over a flat grid of workgroups; the runtime's entry
+table maps a workgroup to an independent component of the level, a chunk of
+rows (PAR levels; one chunk for SEQ levels) and 64 stocks (one lane per
+stock, TS rows coalesced).
   * every value read through a window keeps a per-lane history ring laid
     out [slot][lane] (bank-conflict free), slot count a power of two,
     NaN-initialised -- the reference's "no history before start" rule
@@ -33,6 +34,16 @@ MAX_STAGE_BUFS = 256
 LDS_LIMIT = 160 * 1024
 # LDS per 64-stock workgroup before history rings spill to HBM scratch
 LDS_BUDGET = int(os.environ.get("KUN_LDS_BUDGET", 40 * 1024))
+# registers of prefetched loads per sequential component, and the deepest
+# prefetch (rows) it may use
+SEQ_PREFETCH_REGS = int(os.environ.get("KUN_SEQ_PREFETCH_REGS", 64))
+SEQ_PREFETCH_MAX = int(os.environ.get("KUN_SEQ_PREFETCH_MAX", 8))
+# rows of HBM loads kept in flight by row loops that are not unrolled, and
+# the registers they may take
+ROW_PREFETCH = int(os.environ.get("KUN_ROW_PREFETCH", 4))
+ROW_PREFETCH_REGS = int(os.environ.get("KUN_ROW_PREFETCH_REGS", 48))
+# unroll factor of window loops
+LOOP_UNROLL = int(os.environ.get("KUN_LOOP_UNROLL", 4))
 
 _BIN = {"Add": "+", "Sub": "-", "Mul": "*", "Div": "/"}
 _CMP = {"GreaterThan": ">", "GreaterEqual": ">=", "LessThan": "<", "LessEqual": "<=", "Equals": "=="}
@@ -41,6 +52,12 @@ _CONST_BIN = {"AddConst": "+", "SubConst": "-", "MulConst": "*", "DivConst": "/"
 _UNARY = {"Sqrt": "vsqrt", "Abs": "vabs", "Sign": "vsign", "Not": "bnot", "Exp": "exp_fast"}
 _BINFN = {"Max": "vmax", "Min": "vmin", "Or": "bor", "And": "band"}
 _LAG_OPS = ("BackRef", "FastWindowedSum", "WindowedLinearRegression")
+# rough VALU instructions per evaluation, for balancing work across workgroups
+_OP_COST = {"Div": 10, "DivConst": 10, "Sqrt": 8, "Log": 24, "Exp": 24, "IterValue": 2, "Select": 2,
+            "ReduceAdd": 4, "ReduceMin": 4, "ReduceMax": 4, "ReduceArgMax": 7, "ReduceArgMin": 7,
+            "ReduceRank": 6, "ReduceDecayLinear": 2, "FastWindowedSum": 16, "ExpMovingAvg": 8,
+            "WindowedLinearRegression": 30, "SkipListState": 60, "ForeachBackWindow": 2}
+MAX_COMPS = 64
 
 
 def c_literal(v, T: str) -> str:
@@ -251,7 +268,7 @@ class _Lowering:
             else:
                 raise RuntimeError(f"HIP emitter: unsupported reduction {name}")
         L.append("{")
-        L.append("#pragma unroll 4")
+        L.append(f"#pragma unroll {LOOP_UNROLL}")
         L.append(f"for (int iter = {w - 1}; iter >= 0; --iter) {{")
         for op in body:
             L.extend("  " + x for x in self.op_lines(op))
@@ -556,16 +573,40 @@ class CompEmitter(_Lowering):
                 n_i += w + 1
         self.n_t, self.n_i = n_t, n_i
         self.halo = self._halo() if kind == PAR else 0
+        self.cost = self._cost()
+        self.emit_stores = True
         nops = len(self.computed)
         has_loop = any(isinstance(op, ForeachBackWindow) for op in self.ops)
+        self.depth = 1
         if kind == PAR or has_loop or nops > 60:
             self.prefetch = 1
-        elif nops > 24:
-            self.prefetch = 4
+            nload = sum(1 for op in self.srcs if self._src_row_used(op)) + len(self.lags)
+            self.depth = max(1, min(ROW_PREFETCH, ROW_PREFETCH_REGS // max(1, nload)))
         else:
-            self.prefetch = 8
+            # sequential walks are latency-bound: keep about SEQ_PREFETCH_REGS
+            # registers of loads in flight (the row body is emitted P times)
+            nload = max(1, sum(1 for op in self.srcs if self._src_row_used(op)) + len(self.lags))
+            p = max(2, min(SEQ_PREFETCH_MAX, SEQ_PREFETCH_REGS // nload))
+            if nops > 24:
+                p = min(p, 4)
+            self.prefetch = p
 
     # -- planning helpers -------------------------------------------------
+    def halo_prefill_rows(self) -> int:
+        return max(self.src_ring.values(), default=0)
+
+    def _cost(self) -> int:
+        """Estimated VALU instructions per row (loop bodies count once per
+        window iteration)."""
+        total = 0
+        for op in self.ops:
+            if isinstance(op, (Input, ConstantOp)):
+                continue
+            c = _OP_COST.get(op.__class__.__name__, 1)
+            loop = op.get_loop() if isinstance(op, ReductionOp) else op.get_parent()
+            total += c * (loop.attrs["window"] if loop is not None else 1)
+        return max(1, total)
+
     def _halo(self) -> int:
         """Rows of history a PAR chunk recomputes before its first stored row:
         the longest chain of in-component window offsets."""
@@ -623,7 +664,9 @@ class CompEmitter(_Lowering):
         return f"m{k}[(size_t)({row}) * A.S + s]"
 
     def lag_load(self, op, row: str) -> str:
-        return f"(({row}) >= 0 ? {self.src_load(op, row)} : kun::qnan<{self.T}>())"
+        """Row `row` of a row source, NaN before the run start; the load is
+        unconditional (row clamped at 0) so the row loop has no branch."""
+        return f"kun::nan_before({self.src_load(op, f'(({row}) < 0 ? 0 : ({row}))')}, {row})"
 
     def win(self, src, off) -> str:
         k = self.idx[src]
@@ -640,17 +683,16 @@ class CompEmitter(_Lowering):
     def input_expr(self, op) -> str:
         raise AssertionError("row sources are loaded at the top of the row")
 
-    def store_guard(self) -> str:
-        return "if (i >= c0) " if self.kind == PAR else ""
-
     def output_lines(self, op, val: str) -> List[str]:
+        if not self.emit_stores:
+            return []
         b = self.m.buffer(op.attrs["name"])
-        return [f"{self.store_guard()}b{b.idx}[kun::at<{self.buf_layout(op)}>(i, s, A.S, A.L, A.blk)] = {val};"]
+        return [f"b{b.idx}[kun::at<{self.buf_layout(op)}>(i, s, A.S, A.L, A.blk)] = {val};"]
 
     def after_value(self, k: int) -> List[str]:
         L = self.ring_push(k)
-        if self.st.func.ops[k] in self.plan.mat:
-            L.append(f"{self.store_guard()}w{k}[(size_t)i * A.S + s] = v{k};")
+        if self.emit_stores and self.st.func.ops[k] in self.plan.mat:
+            L.append(f"w{k}[(size_t)i * A.S + s] = v{k};")
         return L
 
     # -- code -------------------------------------------------------------
@@ -698,39 +740,49 @@ class CompEmitter(_Lowering):
         return False
 
     def row_lines(self, q: int, P: int) -> List[str]:
-        """Statements of one row `i` using prefetch slot q."""
+        """Statements of one row `i`. With the row body unrolled P > 1 times,
+        prefetch slot q holds this row and is refilled P rows ahead; with
+        P == 1 the slots form a shift register `depth` rows deep."""
         T = self.T
         L = []
-        nxt = f"(i + {P} < c1 ? i + {P} : c1 - 1)"
+        D = P if P > 1 else self.depth
+        nxt = f"(i + {D} < c1 ? i + {D} : c1 - 1)"
+
+        def take(name: str, load: str) -> str:
+            if P > 1:
+                return f"{name}{q}; {name}{q} = {load};"
+            moves = "".join(f" {name}{j} = {name}{j + 1};" for j in range(D - 1))
+            return f"{name}0;{moves} {name}{D - 1} = {load};"
+
         for op in self.srcs:
             k = self.idx[op]
             if not self._src_row_used(op):
                 continue
-            L.append(f"const {T} v{k} = pf{q}_{k}; pf{q}_{k} = {self.src_load(op, nxt)};")
+            L.append(f"const {T} v{k} = " + take(f"pf{k}_", self.src_load(op, nxt)))
             if k in self.src_ring:
                 L.extend(self.ring_push(k))
         for (k, off) in self.lags:
             op = self.st.func.ops[k]
-            L.append(f"const {T} lg{k}_{off} = pl{q}_{k}_{off}; "
-                     f"pl{q}_{k}_{off} = {self.lag_load(op, f'{nxt} - {off}')};")
+            L.append(f"const {T} lg{k}_{off} = " + take(f"pl{k}_{off}_", self.lag_load(op, f"{nxt} - {off}")))
         L.extend(self.step_lines(skip=set(self.srcs) | self.single))
         return L
 
-    def body(self, zoff: int) -> List[str]:
-        """Statements of the component, run by one lane for one stock."""
+    def body(self) -> List[str]:
+        """Statements of the component, run by one lane for one stock; `e`
+        is this workgroup's entry of the launch table, `chunk_id` its chunk."""
         T = self.T
         L = []
         if self.kind == PAR:
-            L.append("const long c0 = (long)blockIdx.y * A.chunk;")
-            L.append("const long c1 = c0 + A.chunk < A.L ? c0 + A.chunk : A.L;")
+            L.append("const long c0 = (long)chunk_id * A.zchunk[e];")
+            L.append("const long c1 = c0 + A.zchunk[e] < A.L ? c0 + A.zchunk[e] : A.L;")
             L.append(f"const long rbeg = c0 - {self.halo} > 0 ? c0 - {self.halo} : 0;")
         else:
             L.append("const long c0 = 0, c1 = A.L, rbeg = 0;")
             L.append("(void)c0;")
         if self.n_t > self.lds_t or self.n_i:
-            L.append("const long col = (long)blockIdx.y * A.S + s;")
-            L.append("const long cst = A.S * (long)gridDim.y;")
-            L.append(f"{T} *__restrict__ stT = ({T} *)((char *)A.scratch + (size_t){zoff} * cst);")
+            L.append("const long col = (long)chunk_id * A.S + s;")
+            L.append("const long cst = A.S * (long)A.znch[e];")
+            L.append(f"{T} *__restrict__ stT = ({T} *)((char *)A.scratch + A.zscr[e]);")
             L.append(f"int32_t *__restrict__ istp = (int32_t *)(stT + (size_t){self.n_t - self.lds_t} * cst);")
         L.extend(self.buffer_ptrs())
         rows = "(size_t)A.L * A.S"
@@ -746,51 +798,80 @@ class CompEmitter(_Lowering):
             L.append(f"const {T} v{self.idx[op]} = b{b.idx}[s];")
         for k, (slots, base) in sorted(self.ring.items()):
             L.append(f"for (int q = 0; q < {slots}; ++q) {self.tmem(f'{base} + q', base)} = kun::qnan<{T}>();")
-        if self.kind == PAR:
+        if self.kind == PAR and self.src_ring and self.halo_prefill_rows():
+            # history of ringed HBM values before the first computed row:
+            # PF rows of every ringed source per step, loads issued before
+            # the LDS stores so they overlap
+            PF = 8
+            mo = max(self.src_ring.values())
+            L.append(f"for (long rb = rbeg - {mo} > 0 ? rbeg - {mo} : 0; rb < rbeg; rb += {PF}) {{")
+            for k, off in sorted(self.src_ring.items()):
+                op = self.st.func.ops[k]
+                for q in range(PF):
+                    L.append(f"  const {T} pre{k}_{q} = {self.src_load(op, f'(rb + {q} < rbeg ? rb + {q} : rbeg - 1)')};")
             for k, off in sorted(self.src_ring.items()):
                 slots, base = self.ring[k]
-                op = self.st.func.ops[k]
-                L.append(f"for (long r = rbeg - {off} > 0 ? rbeg - {off} : 0; r < rbeg; ++r) "
-                         f"{self.tmem(f'{base} + ((int)r & {slots - 1})', base)} = {self.src_load(op, 'r')};")
+                for q in range(PF):
+                    L.append(f"  if (rb + {q} < rbeg && rb + {q} >= rbeg - {off}) "
+                             f"{self.tmem(f'{base} + ((int)(rb + {q}) & {slots - 1})', base)} = pre{k}_{q};")
+            L.append("}")
         L.extend(self.state_decls())
         P = self.prefetch
-        for q in range(P):
+        for q in range(P if P > 1 else self.depth):
             first = f"(rbeg + {q} < c1 ? rbeg + {q} : c1 - 1)"
             for op in self.srcs:
                 if self._src_row_used(op):
-                    L.append(f"{T} pf{q}_{self.idx[op]} = {self.src_load(op, first)};")
+                    L.append(f"{T} pf{self.idx[op]}_{q} = {self.src_load(op, first)};")
             for (k, off) in self.lags:
-                L.append(f"{T} pl{q}_{k}_{off} = {self.lag_load(self.st.func.ops[k], f'{first} - {off}')};")
-        if P == 1:
-            L.append("for (long i = rbeg; i < c1; ++i) {")
+                L.append(f"{T} pl{k}_{off}_{q} = {self.lag_load(self.st.func.ops[k], f'{first} - {off}')};")
+        if self.kind == PAR:
+            # halo rows rebuild history and store nothing; stored rows follow
+            # in a loop of their own so neither loop branches on the row
+            self.emit_stores = False
+            halo_rows = self.row_lines(0, 1)
+            self.emit_stores = True
+            L.append("for (long i = rbeg; i < c0; ++i) {")
+            L.extend("  " + x for x in halo_rows)
+            L.append("}")
+            L.append("for (long i = c0; i < c1; ++i) {")
+            L.extend("  " + x for x in self.row_lines(0, 1))
+            L.append("}")
+        elif P == 1:
+            L.append("for (long i = 0; i < c1; ++i) {")
             L.extend("  " + x for x in self.row_lines(0, 1))
             L.append("}")
         else:
-            L.append(f"for (long i0 = rbeg; i0 < c1; i0 += {P}) {{")
+            # full groups of P rows without a guard, then the tail
+            L.append("long i0 = 0;")
+            L.append(f"for (; i0 + {P} <= c1; i0 += {P}) {{")
             for q in range(P):
                 L.append(f"  {{ const long i = i0 + {q};")
-                L.append("  if (i < c1) {")
                 L.extend("    " + x for x in self.row_lines(q, P))
-                L.append("  } }")
+                L.append("  }")
             L.append("}")
+            for q in range(P - 1):
+                L.append(f"if (i0 + {q} < c1) {{ const long i = i0 + {q};")
+                L.extend("  " + x for x in self.row_lines(q, P))
+                L.append("}")
         return L
 
 
 class LevelKernel:
-    """One kernel of a batch stage: the components of one plan level."""
+    """One kernel of a batch stage: components of one plan level. The launch
+    is one flat grid: the runtime's entry table (StageArgs.z*) gives each
+    component its own row chunking (balanced by estimated cost, heaviest
+    first) and maps a workgroup to (entry, chunk, 64-stock block)."""
 
-    def __init__(self, module: ModuleIR, stage, plan: StagePlan, level, kname: str, lds_budget: int = None):
+    def __init__(self, module: ModuleIR, stage, plan: StagePlan, level, kname: str, lds_budget: int = None,
+                 comps=None):
         self.kname = kname
         self.kind = level.kind
         budget = LDS_BUDGET if lds_budget is None else lds_budget
-        self.comps = [CompEmitter(module, stage, plan, c, level.kind, budget) for c in level.comps]
+        comps = level.comps if comps is None else comps
+        if len(comps) > MAX_COMPS:
+            raise RuntimeError(f"stage {stage.name}: {len(comps)} components in one kernel (limit {MAX_COMPS})")
+        self.comps = [CompEmitter(module, stage, plan, c, level.kind, budget) for c in comps]
         self.T = self.comps[0].T
-        self.zoff = []
-        off = 0
-        for c in self.comps:
-            self.zoff.append(off)
-            off += c.spill_bytes
-        self.spill_bytes = off
         self.halo = max(c.halo for c in self.comps)
         self.lds_bytes = max(c.lds_bytes for c in self.comps)
         # algorithmic traffic: each distinct row source read once, each
@@ -805,6 +886,14 @@ class LevelKernel:
                 elif op in c.computed and op in plan.mat:
                     io.add(("m", c.idx[op]))
         self.io_values = len(io)
+        # module buffers read / written, and materialised values produced
+        self.reads = sorted({module.buffer(op.attrs["name"]).idx for c in self.comps for op in c.ops
+                             if isinstance(op, Input)})
+        self.writes = sorted({module.buffer(op.attrs["name"]).idx for c in self.comps for op in c.ops
+                              if isinstance(op, Output)})
+        self.produced = {op for c in self.comps for op in c.computed if op in plan.mat}
+        self.consumed = {op for c in self.comps for op in c.ext}
+        self.kdeps: List[int] = []
         if self.lds_bytes > LDS_LIMIT:
             raise RuntimeError(f"stage {stage.name}: {self.lds_bytes} bytes of LDS exceed the gfx950 limit")
 
@@ -812,29 +901,24 @@ class LevelKernel:
         T = self.T
         L = [f'extern "C" __global__ __launch_bounds__({TS_BLOCK}) void {self.kname}(kun::StageArgs A) {{',
              "  const int lane = threadIdx.x;",
-             f"  const long s = (long)blockIdx.x * {TS_BLOCK} + lane;"]
+             "  const int wg = blockIdx.x;",
+             "  int e = 0;",
+             "  while (e + 1 < A.nz && wg >= A.zfirst[e + 1]) ++e;",
+             "  const int nsb = (int)((A.S + 63) >> 6);",
+             "  const int local = wg - A.zfirst[e];",
+             "  const int chunk_id = local / nsb;",
+             f"  const long s = (long)(local - chunk_id * nsb) * {TS_BLOCK} + lane;"]
         lds_t = max(c.lds_t for c in self.comps)
         if lds_t:
             L.append(f"  __shared__ {T} lds[{lds_t * TS_BLOCK}];")
-        else:
-            L.append("  (void)lane;")
         L.append("  if (s >= A.S) return;")
-        multi = len(self.comps) > 1
-        if multi:
-            L.append("  switch (blockIdx.z) {")
+        L.append("  switch (A.zcomp[e]) {")
         for z, c in enumerate(self.comps):
-            body = c.body(self.zoff[z])
-            if multi:
-                L.append(f"  case {z}: {{")
-                L.extend("    " + x for x in body)
-                L.append("  } break;")
-            else:
-                L.append("  {")
-                L.extend("    " + x for x in body)
-                L.append("  }")
-        if multi:
-            L.append("  default: break;")
-            L.append("  }")
+            L.append(f"  case {z}: {{")
+            L.extend("    " + x for x in c.body())
+            L.append("  } break;")
+        L.append("  default: break;")
+        L.append("  }")
         L.append("}")
         return "\n".join(L)
 
@@ -864,11 +948,14 @@ def emit_module(module: ModuleIR, lds_budget: int = None) -> str:
         else:
             plan = stage_plan(module, st)
             kerns = []
-            for j, level in enumerate(plan.levels):
-                lk = LevelKernel(module, st, plan, level, f"{kname}_k{j}", lds_budget)
-                parts.append(lk.source())
-                parts.append("")
-                kerns.append(lk)
+            for level in plan.levels:
+                for g in range(0, len(level.comps), MAX_COMPS):
+                    lk = LevelKernel(module, st, plan, level, f"{kname}_k{len(kerns)}", lds_budget,
+                                     level.comps[g:g + MAX_COMPS])
+                    parts.append(lk.source())
+                    parts.append("")
+                    lk.kdeps = sorted({j for j, prev in enumerate(kerns) if prev.produced & lk.consumed})
+                    kerns.append(lk)
             meta.append((st, kerns, None, None, 0, len(plan.mat)))
     kinds = {"INPUT": 0, "OUTPUT": 1, "TEMP": 2}
     lay = {"STs": 0, "TS": 1, "STREAM": 2}
@@ -883,10 +970,18 @@ def emit_module(module: ModuleIR, lds_budget: int = None) -> str:
             body = ", ".join(str(x) for x in arr) if arr else "-1"
             parts.append(f"static const int kun_{mid}_s{st.idx}_{tag}[] = {{{body}}};")
         if kerns:
+            for lk in kerns:
+                for tag, vals in (("cost", [c.cost for c in lk.comps]), ("halo", [c.halo for c in lk.comps]),
+                                  ("spill", [c.spill_bytes for c in lk.comps]), ("reads", lk.reads),
+                                  ("writes", lk.writes), ("kdeps", lk.kdeps)):
+                    body = ", ".join(str(v) for v in vals) if vals else "-1"
+                    parts.append(f"static const int {lk.kname}_{tag}[] = {{{body}}};")
             parts.append(f"static const KunKernelDesc kun_{mid}_s{st.idx}_kerns[] = {{")
             for lk in kerns:
                 parts.append(f"  {{(const void *)&{lk.kname}, {lk.kind}, {len(lk.comps)}, {lk.halo}, "
-                             f"{lk.lds_bytes}, {lk.spill_bytes}, {lk.io_values}}},")
+                             f"{lk.lds_bytes}, {lk.io_values}, {lk.kname}_cost, {lk.kname}_halo, {lk.kname}_spill, "
+                             f"{lk.kname}_reads, {len(lk.reads)}, {lk.kname}_writes, {len(lk.writes)}, "
+                             f"{lk.kname}_kdeps, {len(lk.kdeps)}}},")
             parts.append("};")
     if module.stream_mode:
         parts.append(f"static const void *kun_{mid}_inits[] = {{"

@@ -130,13 +130,21 @@ __global__ __launch_bounds__(BLOCK) void rank_rows(kun_cs::View in, kun_cs::View
 
 // SCALE_ROWS rows per workgroup; the sum of each row is one sequential chain
 // (lane r of the first wave owns row r), fed from LDS tiles that the whole
-// workgroup loads one tile ahead. Adding +0 for NaN cells leaves a sum of
-// non-negative terms bit-identical, so the chain is one add per stock.
+// workgroup loads one tile ahead and stores already masked: |x|, or +0 for a
+// NaN cell -- adding +0 to a sum of non-negative terms leaves it
+// bit-identical, so the chain is one dependent add per stock. LIN / LOUT are
+// the layouts (-1: read from the view at run time).
 constexpr int SCALE_BLOCK = 256;
 constexpr int SCALE_ROWS = 8;
 constexpr int SCALE_TILE = 1024;
 
-template <typename T>
+template <int LAY> __device__ __forceinline__ size_t lay_at(const kun_cs::View &v, long row, long s, long S, int blk) {
+    if (LAY == 1) return (size_t)row * S + s;
+    if (LAY == 2) return (size_t)s;
+    return cs_at(v.layout, row, s, S, v.rows, blk);
+}
+
+template <typename T, int LIN, int LOUT>
 __global__ __launch_bounds__(SCALE_BLOCK) void scale_rows(kun_cs::View in, kun_cs::View out, long S, long rows,
                                                            int blk) {
     constexpr int PER = SCALE_ROWS * SCALE_TILE / SCALE_BLOCK;
@@ -153,7 +161,7 @@ __global__ __launch_bounds__(SCALE_BLOCK) void scale_rows(kun_cs::View in, kun_c
             const int e = threadIdx.x + q * SCALE_BLOCK;
             const int r = e / SCALE_TILE, c = e % SCALE_TILE;
             const long s = c0 + c;
-            reg[q] = (r < nr && s < S) ? src[cs_at(in.layout, r0 + r + in.row0, s, S, in.rows, blk)] : T(0);
+            reg[q] = (r < nr && s < S) ? src[lay_at<LIN>(in, r0 + r + in.row0, s, S, blk)] : T(0);
         }
     };
     load(0);
@@ -163,7 +171,8 @@ __global__ __launch_bounds__(SCALE_BLOCK) void scale_rows(kun_cs::View in, kun_c
 #pragma unroll
         for (int q = 0; q < PER; q++) {
             const int e = threadIdx.x + q * SCALE_BLOCK;
-            tile[(e / SCALE_TILE) * (SCALE_TILE + 1) + e % SCALE_TILE] = reg[q];
+            const T x = reg[q];
+            tile[(e / SCALE_TILE) * (SCALE_TILE + 1) + e % SCALE_TILE] = (x == x) ? kun::vabs(x) : T(0);
         }
         __syncthreads();
         if (c0 + SCALE_TILE < S) load(c0 + SCALE_TILE);
@@ -172,10 +181,7 @@ __global__ __launch_bounds__(SCALE_BLOCK) void scale_rows(kun_cs::View in, kun_c
             const T *row = tile + threadIdx.x * (SCALE_TILE + 1);
             // in stock order, as the reference's scalar loop
 #pragma unroll 16
-            for (int k = 0; k < n; k++) {
-                const T x = row[k];
-                sum = sum + ((x == x) ? kun::vabs(x) : T(0));
-            }
+            for (int k = 0; k < n; k++) sum = sum + row[k];
         }
     }
     if ((int)threadIdx.x < nr) total[threadIdx.x] = sum;
@@ -184,11 +190,23 @@ __global__ __launch_bounds__(SCALE_BLOCK) void scale_rows(kun_cs::View in, kun_c
         const T tot = total[r];
         const long rr = r0 + r;
         for (long s = threadIdx.x; s < S; s += SCALE_BLOCK) {
-            const T x = src[cs_at(in.layout, rr + in.row0, s, S, in.rows, blk)];
-            dst[cs_at(out.layout, rr + out.row0, s, S, out.rows, blk)] =
-                (x == T(0) && tot == T(0)) ? kun::qnan<T>() : x / tot;
+            const T x = src[lay_at<LIN>(in, rr + in.row0, s, S, blk)];
+            dst[lay_at<LOUT>(out, rr + out.row0, s, S, blk)] = (x == T(0) && tot == T(0)) ? kun::qnan<T>() : x / tot;
         }
     }
+}
+
+template <typename T>
+hipError_t launch_scale_t(const kun_cs::View &in, const kun_cs::View &out, long S, long rows, int blk,
+                          hipStream_t st) {
+    const unsigned grid = (unsigned)((rows + SCALE_ROWS - 1) / SCALE_ROWS);
+    if (in.layout == 1 && out.layout == 1)
+        hipLaunchKernelGGL((scale_rows<T, 1, 1>), dim3(grid), dim3(SCALE_BLOCK), 0, st, in, out, S, rows, blk);
+    else if (in.layout == 2 && out.layout == 2)
+        hipLaunchKernelGGL((scale_rows<T, 2, 2>), dim3(grid), dim3(SCALE_BLOCK), 0, st, in, out, S, rows, blk);
+    else
+        hipLaunchKernelGGL((scale_rows<T, -1, -1>), dim3(grid), dim3(SCALE_BLOCK), 0, st, in, out, S, rows, blk);
+    return hipGetLastError();
 }
 
 template <typename T, int BLOCK, int ITEMS>
@@ -224,12 +242,7 @@ hipError_t launch_rank(int dtype, const View &in, const View &out, long S, long 
 
 hipError_t launch_scale(int dtype, const View &in, const View &out, long S, long rows, int blk, hipStream_t st) {
     if (rows <= 0) return hipSuccess;
-    const unsigned grid = (unsigned)((rows + SCALE_ROWS - 1) / SCALE_ROWS);
-    if (dtype)
-        hipLaunchKernelGGL(scale_rows<double>, dim3(grid), dim3(SCALE_BLOCK), 0, st, in, out, S, rows, blk);
-    else
-        hipLaunchKernelGGL(scale_rows<float>, dim3(grid), dim3(SCALE_BLOCK), 0, st, in, out, S, rows, blk);
-    return hipGetLastError();
+    return dtype ? launch_scale_t<double>(in, out, S, rows, blk, st) : launch_scale_t<float>(in, out, S, rows, blk, st);
 }
 
 } // namespace kun_cs

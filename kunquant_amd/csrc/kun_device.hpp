@@ -321,6 +321,10 @@ template <typename T, int W> struct SortedWin {
     }
 };
 
+// `v` loaded from row `row` (clamped at 0 by the caller) is NaN when the row
+// lies before the run start -- no history before start (Ops.hpp:87-92)
+template <typename T> __device__ __forceinline__ T nan_before(T v, long row) { return row >= 0 ? v : qnan<T>(); }
+
 // element (row, stock) of a buffer in TS [rows][S] or STs [S/b][rows][b]
 template <int LAYOUT> __device__ __forceinline__ size_t at(long row, long s, long S, long rows, int blk) {
     if (LAYOUT == 1) return (size_t)row * S + s;
@@ -333,6 +337,10 @@ template <int LAYOUT> __device__ __forceinline__ size_t at(long row, long s, lon
 
 // Arguments of a batch stage kernel, passed by value: the stage's buffers in
 // the order of its descriptor (inputs, then outputs).
+#ifndef KUN_MAX_COMPS
+#define KUN_MAX_COMPS 64
+#endif
+
 struct StageArgs {
     void *bufs[KUN_MAX_STAGE_BUFS];
     void *scratch;     // HBM columns of rings that do not fit in LDS, sorted windows
@@ -341,8 +349,16 @@ struct StageArgs {
     long T;            // rows of the user inputs
     long start;        // first row of user inputs to compute
     long L;            // rows to compute
-    long chunk;        // rows per chunk of a PAR kernel (grid y)
     int blk;           // STs blocking length
+    // launch table: entry e covers workgroups [zfirst[e], zfirst[e+1]) and
+    // runs component zcomp[e] over znch[e] chunks of zchunk[e] rows, each
+    // chunk x 64-stock block one workgroup; its scratch starts at zscr[e]
+    int nz;
+    int zcomp[KUN_MAX_COMPS];
+    int zfirst[KUN_MAX_COMPS + 1];
+    int zchunk[KUN_MAX_COMPS];
+    int znch[KUN_MAX_COMPS];
+    long zscr[KUN_MAX_COMPS];
 };
 
 } // namespace kun

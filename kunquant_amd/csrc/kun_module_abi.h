@@ -8,7 +8,7 @@
 #ifndef KUN_MODULE_ABI_H
 #define KUN_MODULE_ABI_H
 
-#define KUN_MODULE_ABI_VERSION 0x4B554E03u
+#define KUN_MODULE_ABI_VERSION 0x4B554E05u
 
 enum { KUN_BUF_INPUT = 0, KUN_BUF_OUTPUT = 1, KUN_BUF_TEMP = 2 };
 enum { KUN_STAGE_TS = 0, KUN_STAGE_RANK = 1, KUN_STAGE_SCALE = 2 };
@@ -23,18 +23,28 @@ typedef struct {
     int window;     /* rows of history consumers need */
 } KunBufferDesc;
 
-/* One kernel of a batch time-series stage. PAR kernels split the rows into
- * chunks along grid y (each chunk recomputes `halo` rows of history before
- * its first stored row); SEQ kernels walk every row in order. Grid z selects
- * one of `ncomp` independent op groups. */
+/* One kernel of a batch time-series stage: `ncomp` independent op groups
+ * (components) in one flat grid. PAR kernels split each component's rows
+ * into chunks (each chunk recomputes `comp_halo` rows of history before its
+ * first stored row); SEQ kernels walk every row in order. The runtime sizes
+ * chunks from `comp_cost` (estimated instructions per row) and passes the
+ * launch table in kun::StageArgs. */
 typedef struct {
     const void *kernel; /* host stub */
     int kind;           /* KUN_KERNEL_* */
     int ncomp;
-    int halo;
+    int halo;           /* largest comp_halo */
     int lds_bytes;      /* static LDS per workgroup */
-    int spill_bytes;    /* HBM scratch bytes per (stock, chunk) column */
     int io_values;      /* distinct HBM values read or written per (stock, row) */
+    const int *comp_cost;
+    const int *comp_halo;
+    const int *comp_spill; /* HBM scratch bytes per (stock, chunk) column */
+    const int *reads;      /* module buffers the kernel reads */
+    int nreads;
+    const int *writes;     /* module buffers the kernel writes */
+    int nwrites;
+    const int *kdeps;      /* kernels of the same stage whose materialised values it reads */
+    int nkdeps;
 } KunKernelDesc;
 
 typedef struct {

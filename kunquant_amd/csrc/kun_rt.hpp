@@ -13,6 +13,10 @@
 
 #include "kun_module_abi.h"
 
+#ifndef KUN_MAX_COMPS
+#define KUN_MAX_COMPS 64
+#endif
+
 namespace kunrt {
 
 struct Error : std::runtime_error {
@@ -43,22 +47,33 @@ struct Pool {
 // per kernel, its row chunking and HBM columns.
 struct BatchLayout {
     struct Kern {
-        long chunks = 1, chunk_rows = 1;
-        size_t scratch = 0;
+        size_t scratch = 0;       // arena offset of the kernel's scratch
+        unsigned long blocks = 0; // workgroups of the flat grid
+        int nz = 0;               // launch table (kun::StageArgs z* fields)
+        int zcomp[KUN_MAX_COMPS] = {}, zfirst[KUN_MAX_COMPS + 1] = {}, zchunk[KUN_MAX_COMPS] = {},
+            znch[KUN_MAX_COMPS] = {};
+        long zscr[KUN_MAX_COMPS] = {};
     };
     std::vector<size_t> stage_mat;
     std::vector<std::vector<Kern>> kern;
     size_t bytes = 0;
+    // launch DAG: one node per stage kernel, in a topological order
+    struct Node {
+        int stage = 0, kern = 0, lane = 0;
+        std::vector<int> deps;
+        double est = 0; // estimated microseconds
+    };
+    std::vector<Node> nodes;
 };
 // How PAR kernels split their rows (defaults overridable by the environment
 // variables KUN_TARGET_WORKGROUPS, KUN_HALO_FACTOR, KUN_MIN_CHUNK_ROWS).
 struct ChunkPolicy {
-    long target_workgroups = 4096;
+    long target_workgroups = 32768;
     long halo_factor = 3;
     long min_rows = 64;
     static ChunkPolicy from_env();
 };
-BatchLayout batch_layout(const KunModuleDesc *m, size_t S, size_t L, const ChunkPolicy &pol);
+BatchLayout batch_layout(const KunModuleDesc *m, size_t S, size_t L, const ChunkPolicy &pol, int lanes);
 
 struct DeviceTemps {
     std::vector<void *> bufs;

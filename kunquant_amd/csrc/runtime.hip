@@ -236,22 +236,56 @@ ChunkPolicy ChunkPolicy::from_env() {
     return p;
 }
 
-// Row chunks of a PAR kernel: enough workgroups to fill the chip, each chunk
-// at least `halo_factor` times the halo it recomputes and `min_rows` long.
-static long pick_chunks(const KunKernelDesc &kd, size_t S, size_t L, const ChunkPolicy &pol) {
-    if (kd.kind != KUN_KERNEL_PAR || L == 0) return 1;
-    long nsb = ((long)S + 63) / 64;
-    long rows_min = std::max(std::max(1L, pol.min_rows), pol.halo_factor * (long)kd.halo);
-    long max_ch = std::max(1L, (long)L / rows_min);
-    long want = std::max(1L, (pol.target_workgroups + nsb * kd.ncomp - 1) / (nsb * kd.ncomp));
-    long nch = std::min(max_ch, want);
-    long chunk = ((long)L + nch - 1) / nch;
-    return ((long)L + chunk - 1) / chunk;
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+void plan_nodes(const KunModuleDesc *m, size_t S, size_t L, int lanes, BatchLayout &lay);
+
+// Launch table of one stage kernel. Rows of a PAR component are split into
+// chunks so that every workgroup gets about the same estimated work
+// (comp_cost x rows, halo rows included): the kernel's total work divided by
+// `target_workgroups`, each chunk at least `halo_factor` times the halo it
+// recomputes and `min_rows` long. Entries run heaviest workgroup first.
+static void plan_kernel(const KunKernelDesc &kd, size_t S, size_t L, const ChunkPolicy &pol,
+                        BatchLayout::Kern &k, size_t &off) {
+    const long nsb = ((long)S + 63) / 64;
+    const int nc = kd.ncomp;
+    std::vector<long> nch(nc, 1), rows(nc, (long)L > 0 ? (long)L : 1);
+    if (kd.kind == KUN_KERNEL_PAR && L > 0) {
+        double total = 0;
+        for (int z = 0; z < nc; z++) total += (double)kd.comp_cost[z] * (double)L * (double)nsb;
+        const double per_wg = std::max(1.0, total / (double)std::max(1L, pol.target_workgroups));
+        for (int z = 0; z < nc; z++) {
+            const long h = kd.comp_halo[z];
+            const long rows_min = std::max(std::max(1L, pol.min_rows), pol.halo_factor * h);
+            long lc = (long)(per_wg / (double)kd.comp_cost[z]) - h;
+            lc = std::min((long)L, std::max(rows_min, lc));
+            nch[z] = ((long)L + lc - 1) / lc;
+            rows[z] = ((long)L + nch[z] - 1) / nch[z];
+            nch[z] = ((long)L + rows[z] - 1) / rows[z];
+        }
+    }
+    std::vector<int> order(nc);
+    for (int z = 0; z < nc; z++) order[z] = z;
+    auto wg_work = [&](int z) {
+        const long h = kd.kind == KUN_KERNEL_PAR ? kd.comp_halo[z] : 0;
+        return (double)kd.comp_cost[z] * (double)(rows[z] + h);
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return wg_work(a) > wg_work(b); });
+    k.nz = nc;
+    k.blocks = 0;
+    for (int e = 0; e < nc; e++) {
+        const int z = order[e];
+        k.zcomp[e] = z;
+        k.zfirst[e] = (int)k.blocks;
+        k.zchunk[e] = (int)rows[z];
+        k.znch[e] = (int)nch[z];
+        k.zscr[e] = (long)(off - k.scratch);
+        off += align256((size_t)kd.comp_spill[z] * S * (size_t)nch[z]);
+        k.blocks += (unsigned long)(nsb * nch[z]);
+    }
+    k.zfirst[nc] = (int)k.blocks;
 }
 
-static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
-
-BatchLayout batch_layout(const KunModuleDesc *m, size_t S, size_t L, const ChunkPolicy &pol) {
+BatchLayout batch_layout(const KunModuleDesc *m, size_t S, size_t L, const ChunkPolicy &pol, int lanes) {
     BatchLayout lay;
     const size_t esz = elem_size(m);
     lay.stage_mat.assign(m->nstage, 0);
@@ -265,100 +299,166 @@ BatchLayout batch_layout(const KunModuleDesc *m, size_t S, size_t L, const Chunk
             off += align256((size_t)sd.nmat * L * S * esz);
         }
         for (int j = 0; j < sd.nkern; j++) {
+            if (sd.kerns[j].ncomp > KUN_MAX_COMPS) throw Error("stage kernel with too many components");
             BatchLayout::Kern k;
-            k.chunks = pick_chunks(sd.kerns[j], S, L, pol);
-            k.chunk_rows = L ? ((long)L + k.chunks - 1) / k.chunks : 1;
             k.scratch = off;
-            off += align256((size_t)sd.kerns[j].spill_bytes * S * k.chunks);
+            plan_kernel(sd.kerns[j], S, L, pol, k, off);
             lay.kern[i].push_back(k);
         }
     }
     lay.bytes = off;
+    plan_nodes(m, S, L, lanes, lay);
     return lay;
 }
 
-// Enqueue one stage (all its kernels) on stream `st`.
-static void enqueue_stage(const KunModuleDesc *m, int si, const std::vector<void *> &dev, const BatchLayout &lay,
-                          char *arena, size_t S, size_t T, size_t start, size_t L, hipStream_t st, Executor *prof) {
+// Enqueue kernel `kj` of stage `si` (cross-sectional stages: kj = 0) on `st`.
+static void enqueue_node(const KunModuleDesc *m, int si, int kj, const std::vector<void *> &dev,
+                         const BatchLayout &lay, char *arena, size_t S, size_t T, size_t start, size_t L,
+                         hipStream_t st, Executor *prof) {
     const KunStageDesc &sd = m->stages[si];
-    int nk = sd.kind == KUN_STAGE_TS ? sd.nkern : 1;
-    for (int kj = 0; kj < nk; kj++) {
-        Executor::Pending ev{m, si, kj, nullptr, nullptr};
-        if (prof) {
-            ev.start = prof->take_event();
-            ev.stop = prof->take_event();
-            HIPCHECK(hipEventRecord(ev.start, st));
-        }
-        if (sd.kind == KUN_STAGE_TS) {
-            const KunKernelDesc &kd = sd.kerns[kj];
-            const BatchLayout::Kern &kl = lay.kern[si][kj];
-            kun::StageArgs a;
-            memset(&a, 0, sizeof a);
-            int n = 0;
-            for (int j = 0; j < sd.nin; j++) a.bufs[n++] = dev[sd.in[j]];
-            for (int j = 0; j < sd.nout; j++) a.bufs[n++] = dev[sd.out[j]];
-            a.S = (long)S;
-            a.T = (long)T;
-            a.start = (long)start;
-            a.L = (long)L;
-            a.chunk = kl.chunk_rows;
-            a.blk = m->blocking;
-            a.scratch = kd.spill_bytes ? arena + kl.scratch : nullptr;
-            a.mat = sd.nmat ? arena + lay.stage_mat[si] : nullptr;
-            void *args[] = {&a};
-            dim3 grid((unsigned)((S + sd.block - 1) / sd.block), (unsigned)kl.chunks, (unsigned)kd.ncomp);
-            HIPCHECK(hipLaunchKernel(kd.kernel, grid, dim3(sd.block), args, 0, st));
-        } else {
-            int ib = sd.in[0], ob = sd.out[0];
-            kun_cs::View vi = cs_view(m, ib, dev[ib], T, start, L);
-            kun_cs::View vo = cs_view(m, ob, dev[ob], T, start, L);
-            if (sd.kind == KUN_STAGE_RANK && (long)S > kun_cs::max_rank_stocks(m->dtype))
-                throw Error("Rank over more than " + std::to_string(kun_cs::max_rank_stocks(m->dtype)) +
-                            " stocks is not supported");
-            hipError_t e = sd.kind == KUN_STAGE_RANK
-                               ? kun_cs::launch_rank(m->dtype, vi, vo, (long)S, (long)L, m->blocking, st)
-                               : kun_cs::launch_scale(m->dtype, vi, vo, (long)S, (long)L, m->blocking, st);
-            if (e != hipSuccess) throw Error(std::string("cross-sectional kernel: ") + hipGetErrorString(e));
-        }
-        if (prof) {
-            HIPCHECK(hipEventRecord(ev.stop, st));
-            prof->pending.push_back(ev);
-        }
+    Executor::Pending ev{m, si, kj, nullptr, nullptr};
+    if (prof) {
+        ev.start = prof->take_event();
+        ev.stop = prof->take_event();
+        HIPCHECK(hipEventRecord(ev.start, st));
+    }
+    if (sd.kind == KUN_STAGE_TS) {
+        const KunKernelDesc &kd = sd.kerns[kj];
+        const BatchLayout::Kern &kl = lay.kern[si][kj];
+        kun::StageArgs a;
+        memset(&a, 0, sizeof a);
+        int n = 0;
+        for (int j = 0; j < sd.nin; j++) a.bufs[n++] = dev[sd.in[j]];
+        for (int j = 0; j < sd.nout; j++) a.bufs[n++] = dev[sd.out[j]];
+        a.S = (long)S;
+        a.T = (long)T;
+        a.start = (long)start;
+        a.L = (long)L;
+        a.blk = m->blocking;
+        a.scratch = arena + kl.scratch;
+        a.mat = sd.nmat ? arena + lay.stage_mat[si] : nullptr;
+        a.nz = kl.nz;
+        memcpy(a.zcomp, kl.zcomp, sizeof a.zcomp);
+        memcpy(a.zfirst, kl.zfirst, sizeof a.zfirst);
+        memcpy(a.zchunk, kl.zchunk, sizeof a.zchunk);
+        memcpy(a.znch, kl.znch, sizeof a.znch);
+        memcpy(a.zscr, kl.zscr, sizeof a.zscr);
+        void *args[] = {&a};
+        if (kl.blocks > 0x7fffffffUL) throw Error("batch too large for one launch");
+        HIPCHECK(hipLaunchKernel(kd.kernel, dim3((unsigned)kl.blocks), dim3(sd.block), args, 0, st));
+    } else {
+        int ib = sd.in[0], ob = sd.out[0];
+        kun_cs::View vi = cs_view(m, ib, dev[ib], T, start, L);
+        kun_cs::View vo = cs_view(m, ob, dev[ob], T, start, L);
+        if (sd.kind == KUN_STAGE_RANK && (long)S > kun_cs::max_rank_stocks(m->dtype))
+            throw Error("Rank over more than " + std::to_string(kun_cs::max_rank_stocks(m->dtype)) +
+                        " stocks is not supported");
+        hipError_t e = sd.kind == KUN_STAGE_RANK
+                           ? kun_cs::launch_rank(m->dtype, vi, vo, (long)S, (long)L, m->blocking, st)
+                           : kun_cs::launch_scale(m->dtype, vi, vo, (long)S, (long)L, m->blocking, st);
+        if (e != hipSuccess) throw Error(std::string("cross-sectional kernel: ") + hipGetErrorString(e));
+    }
+    if (prof) {
+        HIPCHECK(hipEventRecord(ev.stop, st));
+        prof->pending.push_back(ev);
     }
 }
 
-// Enqueue every stage of a batch module; dev[b] are device pointers, arena
-// the scratch described by `lay`. Stages run on the executor's lanes (HIP
-// streams) in a topological order, round robin; a stage waits for the
-// stages it reads from through events, and `st` waits for every lane at the
-// end (and every lane for `st` at the start).
+// Launch DAG of a batch run: one node per stage kernel (one per
+// cross-sectional stage), in a topological order. A node depends on the
+// node writing each buffer it reads and on the kernels of its stage whose
+// materialised values it reads. Nodes are list-scheduled over `lanes`
+// streams with rough duration estimates (earliest estimated start first).
+void plan_nodes(const KunModuleDesc *m, size_t S, size_t L, int lanes, BatchLayout &lay) {
+    lay.nodes.clear();
+    std::vector<int> writer(m->nbuf, -1);
+    std::vector<std::vector<int>> node_of(m->nstage);
+    const double nsb = (double)((S + 63) / 64);
+    for (int si : stage_order(m)) {
+        const KunStageDesc &sd = m->stages[si];
+        const int nk = sd.kind == KUN_STAGE_TS ? sd.nkern : 1;
+        for (int kj = 0; kj < nk; kj++) {
+            BatchLayout::Node nd;
+            nd.stage = si;
+            nd.kern = kj;
+            const int id = (int)lay.nodes.size();
+            if (sd.kind == KUN_STAGE_TS) {
+                const KunKernelDesc &kd = sd.kerns[kj];
+                for (int r = 0; r < kd.nreads; r++)
+                    if (writer[kd.reads[r]] >= 0) nd.deps.push_back(writer[kd.reads[r]]);
+                for (int r = 0; r < kd.nkdeps; r++) nd.deps.push_back(node_of[si][kd.kdeps[r]]);
+                for (int w = 0; w < kd.nwrites; w++) writer[kd.writes[w]] = id;
+                // microseconds: PAR ~ total work at ~40% of the VALU peak;
+                // SEQ ~ one dependent walk over the rows
+                double work = 0, seq = 0;
+                for (int z = 0; z < kd.ncomp; z++) {
+                    work += (double)kd.comp_cost[z] * ((double)L + (double)kd.comp_halo[z]) * nsb;
+                    seq = std::max(seq, (double)kd.comp_cost[z]);
+                }
+                nd.est = kd.kind == KUN_KERNEL_PAR ? 5.0 + work * 4.0 / (1024.0 * 2400.0 * 0.4)
+                                                   : 20.0 + (double)L * seq * 8.0 / 2400.0;
+            } else {
+                if (writer[sd.in[0]] >= 0) nd.deps.push_back(writer[sd.in[0]]);
+                writer[sd.out[0]] = id;
+                nd.est = 10.0 + (double)L * (sd.kind == KUN_STAGE_RANK ? 0.18 : 0.12) * (double)S / 8192.0;
+            }
+            std::sort(nd.deps.begin(), nd.deps.end());
+            nd.deps.erase(std::unique(nd.deps.begin(), nd.deps.end()), nd.deps.end());
+            node_of[si].push_back(id);
+            lay.nodes.push_back(nd);
+        }
+    }
+    // list scheduling in topological order
+    std::vector<double> lane_free(std::max(1, lanes), 0.0), finish(lay.nodes.size(), 0.0);
+    for (size_t n = 0; n < lay.nodes.size(); n++) {
+        BatchLayout::Node &nd = lay.nodes[n];
+        double ready = 0;
+        int last_dep = -1;
+        for (int d : nd.deps)
+            if (finish[d] >= ready) {
+                ready = finish[d];
+                last_dep = d;
+            }
+        int best = 0;
+        double best_start = 1e300;
+        for (int l = 0; l < (int)lane_free.size(); l++) {
+            double st = std::max(lane_free[l], ready);
+            bool pref = last_dep >= 0 && lay.nodes[last_dep].lane == l;
+            if (st < best_start - 1e-9 || (st <= best_start + 1e-9 && pref)) {
+                best_start = st;
+                best = l;
+            }
+        }
+        nd.lane = best;
+        finish[n] = best_start + nd.est;
+        lane_free[best] = finish[n];
+    }
+}
+
+// Enqueue every kernel of a batch module; dev[b] are device pointers, arena
+// the scratch described by `lay`. Nodes run on the executor's lanes (HIP
+// streams) as planned by plan_nodes; a node waits through events for its
+// dependencies on other lanes, every lane waits for `st` at the start and
+// `st` waits for every lane at the end.
 void enqueue_batch(Executor &ex, const KunModuleDesc *m, const std::vector<void *> &dev, const BatchLayout &lay,
                    char *arena, size_t S, size_t T, size_t start, size_t L, hipStream_t st) {
     if (L == 0) return;
     Executor *prof = ex.profiling ? &ex : nullptr;
-    const std::vector<int> &order = stage_order(m);
     const int nl = (int)ex.lanes.size();
-    if (nl == 0 || m->nstage == 1) {
-        for (int si : order) enqueue_stage(m, si, dev, lay, arena, S, T, start, L, st, prof);
+    if (nl == 0) {
+        for (const auto &nd : lay.nodes) enqueue_node(m, nd.stage, nd.kern, dev, lay, arena, S, T, start, L, st, prof);
         return;
     }
-    ex.ensure_stage_events(m->nstage);
+    ex.ensure_stage_events((int)lay.nodes.size());
     HIPCHECK(hipEventRecord(ex.fork, st));
     for (int l = 0; l < nl; l++) HIPCHECK(hipStreamWaitEvent(ex.lanes[l], ex.fork, 0));
-    std::vector<int> lane_of(m->nstage, 0);
-    int next = 0;
-    for (int si : order) {
-        const KunStageDesc &sd = m->stages[si];
-        const int l = next;
-        next = (next + 1) % nl;
-        lane_of[si] = l;
-        hipStream_t ls = ex.lanes[l];
-        for (int d = 0; d < sd.ndeps; d++) {
-            int dep = sd.deps[d];
-            if (lane_of[dep] != l) HIPCHECK(hipStreamWaitEvent(ls, ex.stage_done[dep], 0));
-        }
-        enqueue_stage(m, si, dev, lay, arena, S, T, start, L, ls, prof);
-        HIPCHECK(hipEventRecord(ex.stage_done[si], ls));
+    for (size_t n = 0; n < lay.nodes.size(); n++) {
+        const auto &nd = lay.nodes[n];
+        hipStream_t ls = ex.lanes[nd.lane % nl];
+        for (int d : nd.deps)
+            if (lay.nodes[d].lane % nl != nd.lane % nl) HIPCHECK(hipStreamWaitEvent(ls, ex.stage_done[d], 0));
+        enqueue_node(m, nd.stage, nd.kern, dev, lay, arena, S, T, start, L, ls, prof);
+        HIPCHECK(hipEventRecord(ex.stage_done[n], ls));
     }
     for (int l = 0; l < nl; l++) {
         HIPCHECK(hipEventRecord(ex.lane_done[l], ex.lanes[l]));
@@ -402,7 +502,7 @@ void run_graph_host(Executor &ex, const KunModuleDesc *m, const NameMap &bufs, s
         if (d.kind == KUN_BUF_INPUT)
             HIPCHECK(hipMemcpyAsync(dev[b], bufs.at(d.name), bytes[b], hipMemcpyHostToDevice, ex.stream));
     }
-    BatchLayout lay = batch_layout(m, S, L, ex.chunking);
+    BatchLayout lay = batch_layout(m, S, L, ex.chunking, (int)ex.lanes.size());
     struct ReleaseArena {
         Executor &ex;
         char *p = nullptr;
@@ -435,7 +535,7 @@ void run_graph_device(Executor &ex, const KunModuleDesc *m, const NameMap &bufs,
     if (it == ex.temps.end()) {
         // temporaries first, then the scratch arena
         DeviceTemps t;
-        t.lay = batch_layout(m, S, L, ex.chunking);
+        t.lay = batch_layout(m, S, L, ex.chunking, (int)ex.lanes.size());
         t.bufs.assign(m->nbuf, nullptr);
         for (int b = 0; b < m->nbuf; b++)
             if (m->bufs[b].kind == KUN_BUF_TEMP) {

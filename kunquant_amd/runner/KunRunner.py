@@ -122,7 +122,7 @@ class Executor:
             raise RuntimeError(_err())
         return ms.value, n.value
 
-    def set_chunking(self, target_workgroups: int = 4096, halo_factor: int = 3, min_rows: int = 64):
+    def set_chunking(self, target_workgroups: int = 32768, halo_factor: int = 3, min_rows: int = 64):
         """How row-chunked stage kernels split the rows (results do not
         depend on it)."""
         if _load().kunExecutorSetChunking(self._h, target_workgroups, halo_factor, min_rows):

@@ -17,18 +17,19 @@ thread_local kun_emu_dim3 blockIdx, threadIdx, gridDim, blockDim;
 typedef void (*StageKernel)(kun::StageArgs);
 
 // bufs: device-layout pointers of every module buffer; chunk_rows: rows per
-// chunk of every row-chunked kernel (0: one chunk).
+// chunk of every row-chunked component (0: one chunk). Components run in
+// descriptor order (the device runtime orders them by cost; results do not
+// depend on the order).
 extern "C" int kun_emu_run_stage(const KunModuleDesc *m, int stage, void **bufs, long S, long T, long start,
                                  long L, long chunk_rows) {
     const KunStageDesc &sd = m->stages[stage];
     if (sd.kind != KUN_STAGE_TS || L <= 0) return 1;
     const size_t esz = m->dtype == KUN_DTYPE_DOUBLE ? 8 : 4;
     std::vector<char> mat((size_t)sd.nmat * L * S * esz + 1);
+    const long nsb = (S + 63) / 64;
     for (int kj = 0; kj < sd.nkern; kj++) {
         const KunKernelDesc &kd = sd.kerns[kj];
-        long chunk = (kd.kind == KUN_KERNEL_PAR && chunk_rows > 0) ? chunk_rows : L;
-        long nch = (L + chunk - 1) / chunk;
-        std::vector<char> scratch((size_t)kd.spill_bytes * S * nch + 1);
+        if (kd.ncomp > KUN_MAX_COMPS) return 2;
         kun::StageArgs a;
         memset(&a, 0, sizeof a);
         int n = 0;
@@ -38,20 +39,33 @@ extern "C" int kun_emu_run_stage(const KunModuleDesc *m, int stage, void **bufs,
         a.T = T;
         a.start = start;
         a.L = L;
-        a.chunk = chunk;
         a.blk = m->blocking;
-        a.scratch = scratch.data();
         a.mat = mat.data();
-        gridDim = {(unsigned)((S + sd.block - 1) / sd.block), (unsigned)nch, (unsigned)kd.ncomp};
+        a.nz = kd.ncomp;
+        size_t off = 0;
+        long blocks = 0;
+        for (int z = 0; z < kd.ncomp; z++) {
+            long chunk = (kd.kind == KUN_KERNEL_PAR && chunk_rows > 0) ? chunk_rows : L;
+            long nch = (L + chunk - 1) / chunk;
+            a.zcomp[z] = z;
+            a.zfirst[z] = (int)blocks;
+            a.zchunk[z] = (int)chunk;
+            a.znch[z] = (int)nch;
+            a.zscr[z] = (long)off;
+            off += (size_t)kd.comp_spill[z] * S * nch;
+            blocks += nsb * nch;
+        }
+        a.zfirst[kd.ncomp] = (int)blocks;
+        std::vector<char> scratch(off + 1);
+        a.scratch = scratch.data();
+        gridDim = {(unsigned)blocks, 1, 1};
         blockDim = {(unsigned)sd.block, 1, 1};
-        for (unsigned z = 0; z < gridDim.z; z++)
-            for (unsigned y = 0; y < gridDim.y; y++)
-                for (unsigned x = 0; x < gridDim.x; x++)
-                    for (unsigned t = 0; t < blockDim.x; t++) {
-                        blockIdx = {x, y, z};
-                        threadIdx = {t, 0, 0};
-                        ((StageKernel)kd.kernel)(a);
-                    }
+        for (unsigned x = 0; x < gridDim.x; x++)
+            for (unsigned t = 0; t < blockDim.x; t++) {
+                blockIdx = {x, 0, 0};
+                threadIdx = {t, 0, 0};
+                ((StageKernel)kd.kernel)(a);
+            }
     }
     return 0;
 }

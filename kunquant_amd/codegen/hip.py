@@ -34,14 +34,10 @@ MAX_STAGE_BUFS = 256
 LDS_LIMIT = 160 * 1024
 # LDS per 64-stock workgroup before history rings spill to HBM scratch
 LDS_BUDGET = int(os.environ.get("KUN_LDS_BUDGET", 40 * 1024))
-# registers of prefetched loads per sequential component, and the deepest
-# prefetch (rows) it may use
-SEQ_PREFETCH_REGS = int(os.environ.get("KUN_SEQ_PREFETCH_REGS", 64))
-SEQ_PREFETCH_MAX = int(os.environ.get("KUN_SEQ_PREFETCH_MAX", 8))
-# rows of HBM loads kept in flight by row loops that are not unrolled, and
-# the registers they may take
-ROW_PREFETCH = int(os.environ.get("KUN_ROW_PREFETCH", 4))
-ROW_PREFETCH_REGS = int(os.environ.get("KUN_ROW_PREFETCH_REGS", 48))
+# registers of prefetched loads per component, and the deepest prefetch
+# (rows) of a small sequential component
+PREFETCH_REGS = int(os.environ.get("KUN_PREFETCH_REGS", 64))
+SEQ_PREFETCH_MAX = int(os.environ.get("KUN_SEQ_PREFETCH_MAX", 16))
 # unroll factor of window loops
 LOOP_UNROLL = int(os.environ.get("KUN_LOOP_UNROLL", 4))
 
@@ -58,6 +54,9 @@ _OP_COST = {"Div": 10, "DivConst": 10, "Sqrt": 8, "Log": 24, "Exp": 24, "IterVal
             "ReduceRank": 6, "ReduceDecayLinear": 2, "FastWindowedSum": 16, "ExpMovingAvg": 8,
             "WindowedLinearRegression": 30, "SkipListState": 60, "ForeachBackWindow": 2}
 MAX_COMPS = 64
+# upper LDS bytes of the launch classes a level's components are split into
+# (16, 8, 4 and 1 workgroups of 64 lanes per CU)
+LDS_CLASSES = (0, 10 * 1024, 20 * 1024, 40 * 1024, LDS_LIMIT)
 
 
 def c_literal(v, T: str) -> str:
@@ -542,7 +541,7 @@ class CompEmitter(_Lowering):
         n_lds = 0
         spilled = []
         for op in sorted(ring_need, key=lambda o: self.idx[o]):
-            slots = _pow2(ring_need[op] + 1)
+            slots = ring_need[op] + 1
             if n_lds + slots <= cap:
                 self.ring[self.idx[op]] = (slots, n_lds)
                 n_lds += slots
@@ -551,7 +550,7 @@ class CompEmitter(_Lowering):
         self.src_ring: Dict[int, int] = {}  # row source idx -> largest offset, ringed in LDS
         self.direct: set = set()            # row sources read by IterValue straight from HBM
         for op in sorted(src_iter, key=lambda o: (-src_iter[o], self.idx[o])):
-            slots = _pow2(src_iter[op] + 1)
+            slots = src_iter[op] + 1
             if n_lds + slots <= cap:
                 self.ring[self.idx[op]] = (slots, n_lds)
                 self.src_ring[self.idx[op]] = src_iter[op]
@@ -577,19 +576,19 @@ class CompEmitter(_Lowering):
         self.emit_stores = True
         nops = len(self.computed)
         has_loop = any(isinstance(op, ForeachBackWindow) for op in self.ops)
+        # Row loops are unrolled P times; copy q owns prefetch slot q, which
+        # always holds the next row r with r % P == q (loaded P rows ahead),
+        # so loads stay in flight across P rows of work. Light row bodies
+        # need deeper prefetch to cover HBM latency.
         self.depth = 1
-        if kind == PAR or has_loop or nops > 60:
-            self.prefetch = 1
-            nload = sum(1 for op in self.srcs if self._src_row_used(op)) + len(self.lags)
-            self.depth = max(1, min(ROW_PREFETCH, ROW_PREFETCH_REGS // max(1, nload)))
+        nload = max(1, sum(1 for op in self.srcs if self._src_row_used(op)) + len(self.lags))
+        if kind == PAR:
+            p = 1 if self.cost >= 1024 else 2 if self.cost >= 256 else 4 if self.cost >= 64 else 8
+        elif has_loop or nops > 60:
+            p = 1
         else:
-            # sequential walks are latency-bound: keep about SEQ_PREFETCH_REGS
-            # registers of loads in flight (the row body is emitted P times)
-            nload = max(1, sum(1 for op in self.srcs if self._src_row_used(op)) + len(self.lags))
-            p = max(2, min(SEQ_PREFETCH_MAX, SEQ_PREFETCH_REGS // nload))
-            if nops > 24:
-                p = min(p, 4)
-            self.prefetch = p
+            p = SEQ_PREFETCH_MAX if nops <= 24 else 4
+        self.prefetch = max(1, min(p, PREFETCH_REGS // nload))
 
     # -- planning helpers -------------------------------------------------
     def halo_prefill_rows(self) -> int:
@@ -640,6 +639,22 @@ class CompEmitter(_Lowering):
         return (self.n_t - self.lds_t) * self.esz + self.n_i * 4
 
     # -- storage hooks ----------------------------------------------------
+    # Rings hold exactly (largest offset + 1) rows; rp<k> is row i's slot,
+    # computed once per row on uniform scalars.
+    def ring_read(self, k: int, off) -> str:
+        slots, base = self.ring[k]
+        q = f"(rp{k} - ({off}))"
+        return self.tmem(f"{base} + ({q} < 0 ? {q} + {slots} : {q})", base)
+
+    def ring_push(self, k: int) -> List[str]:
+        if k not in self.ring:
+            return []
+        slots, base = self.ring[k]
+        return [f"{self.tmem(f'{base} + rp{k}', base)} = v{k};"]
+
+    def ring_positions(self) -> List[str]:
+        return [f"const int rp{k} = (int)i % {slots};" for k, (slots, base) in sorted(self.ring.items())]
+
     def tmem(self, slot, base: int = None) -> str:
         b = base if base is not None else (slot if isinstance(slot, int) else 0)
         if b < self.lds_t:
@@ -747,6 +762,7 @@ class CompEmitter(_Lowering):
         L = []
         D = P if P > 1 else self.depth
         nxt = f"(i + {D} < c1 ? i + {D} : c1 - 1)"
+        L.extend(self.ring_positions())
 
         def take(name: str, load: str) -> str:
             if P > 1:
@@ -813,12 +829,14 @@ class CompEmitter(_Lowering):
                 slots, base = self.ring[k]
                 for q in range(PF):
                     L.append(f"  if (rb + {q} < rbeg && rb + {q} >= rbeg - {off}) "
-                             f"{self.tmem(f'{base} + ((int)(rb + {q}) & {slots - 1})', base)} = pre{k}_{q};")
+                             f"{self.tmem(f'{base} + (int)(rb + {q}) % {slots}', base)} = pre{k}_{q};")
             L.append("}")
         L.extend(self.state_decls())
         P = self.prefetch
-        for q in range(P if P > 1 else self.depth):
-            first = f"(rbeg + {q} < c1 ? rbeg + {q} : c1 - 1)"
+        for q in range(P):
+            # first row at or after rbeg that belongs to slot q
+            row = f"(rbeg + (({q} - rbeg % {P} + {P}) % {P}))" if P > 1 else "rbeg"
+            first = f"({row} < c1 ? {row} : c1 - 1)"
             for op in self.srcs:
                 if self._src_row_used(op):
                     L.append(f"{T} pf{self.idx[op]}_{q} = {self.src_load(op, first)};")
@@ -826,33 +844,45 @@ class CompEmitter(_Lowering):
                 L.append(f"{T} pl{k}_{off}_{q} = {self.lag_load(self.st.func.ops[k], f'{first} - {off}')};")
         if self.kind == PAR:
             # halo rows rebuild history and store nothing; stored rows follow
-            # in a loop of their own so neither loop branches on the row
+            # in a phase of their own so neither phase branches on the row
             self.emit_stores = False
-            halo_rows = self.row_lines(0, 1)
+            L.extend(self._phase("rbeg", "c0", P))
             self.emit_stores = True
-            L.append("for (long i = rbeg; i < c0; ++i) {")
-            L.extend("  " + x for x in halo_rows)
-            L.append("}")
-            L.append("for (long i = c0; i < c1; ++i) {")
-            L.extend("  " + x for x in self.row_lines(0, 1))
-            L.append("}")
-        elif P == 1:
-            L.append("for (long i = 0; i < c1; ++i) {")
-            L.extend("  " + x for x in self.row_lines(0, 1))
-            L.append("}")
+            L.extend(self._phase("c0", "c1", P))
         else:
-            # full groups of P rows without a guard, then the tail
-            L.append("long i0 = 0;")
-            L.append(f"for (; i0 + {P} <= c1; i0 += {P}) {{")
+            L.extend(self._phase("0", "c1", P))
+        return L
+
+    def _phase(self, lo: str, hi: str, P: int) -> List[str]:
+        """Rows [lo, hi) in groups of P aligned to multiples of P (copy q
+        handles rows r % P == q): a guarded first group, unguarded full
+        groups, a guarded tail."""
+        L = []
+        if P == 1:
+            L.append(f"for (long i = {lo}; i < {hi}; ++i) {{")
+            L.extend("  " + x for x in self.row_lines(0, 1))
+            L.append("}")
+            return L
+        L.append("{")
+        L.append(f"  long i0 = {lo} - {lo} % {P};")
+        if lo != "0":
             for q in range(P):
                 L.append(f"  {{ const long i = i0 + {q};")
+                L.append(f"  if (i >= {lo} && i < {hi}) {{")
                 L.extend("    " + x for x in self.row_lines(q, P))
-                L.append("  }")
-            L.append("}")
-            for q in range(P - 1):
-                L.append(f"if (i0 + {q} < c1) {{ const long i = i0 + {q};")
-                L.extend("  " + x for x in self.row_lines(q, P))
-                L.append("}")
+                L.append("  } }")
+            L.append(f"  i0 += {P};")
+        L.append(f"  for (; i0 + {P} <= {hi}; i0 += {P}) {{")
+        for q in range(P):
+            L.append(f"    {{ const long i = i0 + {q};")
+            L.extend("      " + x for x in self.row_lines(q, P))
+            L.append("    }")
+        L.append("  }")
+        for q in range(P - 1):
+            L.append(f"  if (i0 + {q} < {hi}) {{ const long i = i0 + {q};")
+            L.extend("    " + x for x in self.row_lines(q, P))
+            L.append("  }")
+        L.append("}")
         return L
 
 
@@ -948,10 +978,23 @@ def emit_module(module: ModuleIR, lds_budget: int = None) -> str:
         else:
             plan = stage_plan(module, st)
             kerns = []
+            budget = LDS_BUDGET if lds_budget is None else lds_budget
             for level in plan.levels:
-                for g in range(0, len(level.comps), MAX_COMPS):
-                    lk = LevelKernel(module, st, plan, level, f"{kname}_k{len(kerns)}", lds_budget,
-                                     level.comps[g:g + MAX_COMPS])
+                # one launch per LDS class, so that components with small
+                # rings are not held to the occupancy of the largest
+                classes: Dict[int, list] = {}
+                for c in level.comps:
+                    cls = 0
+                    if level.kind == PAR:  # sequential levels are latency-bound: one launch
+                        lds = CompEmitter(module, st, plan, c, level.kind, budget).lds_bytes
+                        cls = next(j for j, lim in enumerate(LDS_CLASSES) if lds <= lim)
+                    classes.setdefault(cls, []).append(c)
+                groups = []
+                for cls in sorted(classes):
+                    cl = classes[cls]
+                    groups.extend(cl[g:g + MAX_COMPS] for g in range(0, len(cl), MAX_COMPS))
+                for group in groups:
+                    lk = LevelKernel(module, st, plan, level, f"{kname}_k{len(kerns)}", lds_budget, group)
                     parts.append(lk.source())
                     parts.append("")
                     lk.kdeps = sorted({j for j, prev in enumerate(kerns) if prev.produced & lk.consumed})

@@ -24,7 +24,7 @@ Stages holding Accumulator webs (or anything the split cannot express) run
 as a single SEQ level: the whole stage walks its rows sequentially."""
 from typing import Dict, List, Optional, Set
 
-from ..Op import ConstantOp, ForeachBackWindow, Input, OpBase, ReductionOp
+from ..Op import ConstantOp, ForeachBackWindow, Input, OpBase, Output, ReductionOp
 
 PAR, SEQ = 0, 1
 
@@ -122,6 +122,35 @@ def plan_stage(ops: List[OpBase], force_seq: bool = False) -> StagePlan:
             lvl[u] = m if m % 2 == 0 else m + 1
         else:
             lvl[u] = m
+    # Sequential levels walk every row on one lane per stock, so they keep
+    # only what their recurrences need: a unit on a sequential level that no
+    # recurrence (or state consumer) of the level depends on moves to the
+    # next level, where rows run in parallel; an Output (or temp output)
+    # moves with its input.
+    def needs(u):
+        return [unit_of[inp] for o in units[u] for inp in o.inputs if unit_of[inp] != u]
+
+    needed = set()
+    for u in reversed(range(len(units))):
+        L = lvl[u]
+        if L is None or L % 2 == 0:
+            continue
+        name = units[u][0].__class__.__name__
+        if name in REC_OPS or name in BOUND_OPS or u in needed:
+            needed.add(u)
+            for v in needs(u):
+                if lvl[v] == L:
+                    needed.add(v)
+    passthrough = ("WindowedTempOutput", "ReturnFirstValue")
+    for u in range(len(units)):
+        L = lvl[u]
+        if L is None or L % 2 == 0 or u in needed or len(units[u]) != 1:
+            continue
+        head = units[u][0]
+        if isinstance(head, Output) or head.__class__.__name__ in passthrough:
+            if lvl[unit_of[head.inputs[0]]] == L:
+                continue
+        lvl[u] = L + 1
     level_of = {op: lvl[unit_of[op]] for op in ops}
     # materialised values: computed on one level, read on a higher one
     mat: Dict[OpBase, int] = {}

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 #include <rocprim/block/block_radix_sort.hpp>
 #include <rocprim/block/block_scan.hpp>
+#include <rocprim/block/block_sort.hpp>
 #include <stdint.h>
 
 #include "kun_cs.hpp"
@@ -45,11 +46,28 @@ __device__ __forceinline__ size_t cs_at(int layout, long row, long s, long S, lo
 // a = the last group start at or before p (a block max-scan of group
 // starts), b = the end recorded by the group's last element. Ranks go to LDS
 // at their stock index and leave in one coalesced pass.
-template <typename T, int BLOCK, int ITEMS>
+template <typename K, int BLOCK, int ITEMS, bool MERGE> struct PairSort;
+template <typename K, int BLOCK, int ITEMS> struct PairSort<K, BLOCK, ITEMS, false> {
+    using Impl = rocprim::block_radix_sort<K, BLOCK, ITEMS, int>;
+    using storage_type = typename Impl::storage_type;
+    __device__ __forceinline__ void sort(K (&k)[ITEMS], int (&v)[ITEMS], storage_type &st) { Impl().sort(k, v, st); }
+};
+template <typename K, int BLOCK, int ITEMS> struct PairSort<K, BLOCK, ITEMS, true> {
+    using Impl = rocprim::block_sort<K, BLOCK, ITEMS, int, rocprim::block_sort_algorithm::merge_sort>;
+    using storage_type = typename Impl::storage_type;
+    __device__ __forceinline__ void sort(K (&k)[ITEMS], int (&v)[ITEMS], storage_type &st) {
+        Impl().sort(k, v, st, [](const K &a, const K &b) { return a < b; });
+    }
+};
+
+// MERGE: rocPRIM's merge sort (faster than its radix sort for large rows);
+// either sort serves, since ranks only need every element inside its tie
+// group.
+template <typename T, int BLOCK, int ITEMS, bool MERGE>
 __global__ __launch_bounds__(BLOCK) void rank_rows(kun_cs::View in, kun_cs::View out, long S, int blk) {
     using K = typename KeyOf<T>::K;
     constexpr int N = BLOCK * ITEMS;
-    using Sort = rocprim::block_radix_sort<K, BLOCK, ITEMS, int>;
+    using Sort = PairSort<K, BLOCK, ITEMS, MERGE>;
     using Scan = rocprim::block_scan<int, BLOCK>;
     static_assert(sizeof(T) <= sizeof(K) && sizeof(int) <= sizeof(K), "LDS aliasing");
     __shared__ union {
@@ -115,7 +133,9 @@ __global__ __launch_bounds__(BLOCK) void rank_rows(kun_cs::View in, kun_cs::View
         if (p < n) {
             const int a = start[j], b = endof[a];
             const int64_t sum = ((int64_t)a + b + 1) * (int64_t)(b - a) / 2;
-            r[j] = (T)sum / (T)(b - a) / (T)n;
+            // x / 1 == x exactly: a group of one skips the first division
+            const T q = (b - a == 1) ? (T)sum : (T)sum / (T)(b - a);
+            r[j] = q / (T)n;
         } else {
             r[j] = kun::qnan<T>();
         }
@@ -209,23 +229,30 @@ hipError_t launch_scale_t(const kun_cs::View &in, const kun_cs::View &out, long 
     return hipGetLastError();
 }
 
-template <typename T, int BLOCK, int ITEMS>
+template <typename T, int BLOCK, int ITEMS, bool MERGE>
 hipError_t launch_rank_n(const kun_cs::View &in, const kun_cs::View &out, long S, long rows, int blk,
                          hipStream_t st) {
-    hipLaunchKernelGGL((rank_rows<T, BLOCK, ITEMS>), dim3((unsigned)rows), dim3(BLOCK), 0, st, in, out, S, blk);
+    hipLaunchKernelGGL((rank_rows<T, BLOCK, ITEMS, MERGE>), dim3((unsigned)rows), dim3(BLOCK), 0, st, in, out, S,
+                       blk);
     return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_rank_t(const kun_cs::View &in, const kun_cs::View &out, long S, long rows, int blk,
                          hipStream_t st) {
-    if (S <= 256) return launch_rank_n<T, 256, 1>(in, out, S, rows, blk, st);
-    if (S <= 512) return launch_rank_n<T, 256, 2>(in, out, S, rows, blk, st);
-    if (S <= 1024) return launch_rank_n<T, 256, 4>(in, out, S, rows, blk, st);
-    if (S <= 2048) return launch_rank_n<T, 256, 8>(in, out, S, rows, blk, st);
-    if (S <= 4096) return launch_rank_n<T, 256, 16>(in, out, S, rows, blk, st);
-    if (S <= 8192) return launch_rank_n<T, 256, 32>(in, out, S, rows, blk, st);
-    if (S <= 16384) return launch_rank_n<T, 512, 32>(in, out, S, rows, blk, st);
+    if (S <= 256) return launch_rank_n<T, 256, 1, false>(in, out, S, rows, blk, st);
+    if (S <= 512) return launch_rank_n<T, 256, 2, false>(in, out, S, rows, blk, st);
+    if (S <= 1024) return launch_rank_n<T, 256, 4, false>(in, out, S, rows, blk, st);
+    if (S <= 2048) return launch_rank_n<T, 1024, 2, true>(in, out, S, rows, blk, st);
+    if (S <= 4096) return launch_rank_n<T, 1024, 4, true>(in, out, S, rows, blk, st);
+    if (S <= 8192) return launch_rank_n<T, 1024, 8, true>(in, out, S, rows, blk, st);
+    if (S <= 16384) {
+        // a 64-bit-key merge sort of 16384 pairs needs more LDS than a CU has
+        if constexpr (sizeof(T) == 8)
+            return launch_rank_n<T, 512, 32, false>(in, out, S, rows, blk, st);
+        else
+            return launch_rank_n<T, 1024, 16, true>(in, out, S, rows, blk, st);
+    }
     return hipErrorInvalidValue;
 }
 

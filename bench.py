@@ -29,7 +29,18 @@ def main():
     ap.add_argument("--days", type=int, default=1024)
     ap.add_argument("--cpu-rows", type=int, default=128, help="rows of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 1 GiB device-to-device three times (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        x = torch.ones(1 << 28, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": 4 << 28, "copies": 3}))
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the factor pipeline "
               f"shards by time or by job across GPUs, which this build does not run.", file=sys.stderr)
@@ -102,6 +113,16 @@ def main():
     kernel_sum = sum(p[0] for p in per_kernel)
     dom_ms, dom_i, dom_j, dom_label, dom_bytes = per_kernel[0]
     achieved = dom_bytes / (dom_ms / 1e3) / 1e9
+    # HBM traffic of the dominant kernel, from rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # passes scaled by the same counters on a plain copy (profiles/*_traffic.json)
+    dom_name = f"kun_{args.module}_s{dom_i}_k{dom_j}"
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"{args.module}_{S}x{T}_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tk = json.load(f).get("kernels", {}).get(dom_name)
+        if tk:
+            traffic = tk["fetch_bytes"] + tk["write_bytes"]
     graph_bytes = (len(inputs) + len(outputs)) * S * T * esz
     graph_gbs = graph_bytes / (ms / 1e3) / 1e9
     by_kind = {}
@@ -126,7 +147,7 @@ def main():
                    "module": args.module, "stocks": S, "days": T, "factors": len(outputs), "inputs": len(inputs),
                    "stages": len(stages)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": f"stage {dom_i} kernel {dom_j} ({dom_label})", "kernel_ms": dom_ms,
                      "algorithmic_bytes": dom_bytes},
         "graph": {"algorithmic_bytes": graph_bytes, "achieved_GBs": graph_gbs,

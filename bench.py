@@ -6,6 +6,12 @@ batch run of the whole graph (every stage kernel) with inputs and outputs
 already resident in HBM. Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--module cfg2_f32|cfg2_f64]
+
+Batched uneven jobs (--workload jobs): 64 independent jobs (stocks uniform
+in 50..6000, days in 250..2500, graphs of 20..60 factors), each job's
+chunking picked by the autotuner, enqueued together in one call; compared
+with the same jobs as sequential plain runs (device-resident, and host
+buffers through runGraph).
 """
 import argparse
 import json
@@ -29,6 +35,10 @@ def main():
     ap.add_argument("--days", type=int, default=1024)
     ap.add_argument("--cpu-rows", type=int, default=128, help="rows of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", choices=("factors", "jobs"), default="factors")
+    ap.add_argument("--jobs", type=int, default=64, help="jobs of --workload jobs")
+    ap.add_argument("--no-host-baseline", action="store_true",
+                    help="--workload jobs: skip the sequential host-buffer runGraph pass")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device-to-device three times (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -45,6 +55,9 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the factor pipeline "
               f"shards by time or by job across GPUs, which this build does not run.", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "jobs":
+        run_jobs(args)
+        return
 
     import numpy as np
     import torch
@@ -171,6 +184,121 @@ def main():
                                   "kind": "port",
                                   "sample": f"same graph and data, first {rows} of {T} days, {S} stocks "
                                             f"({cpu_s:.1f} s on {cores} host threads)"}
+    print(json.dumps(result), flush=True)
+
+
+def _checksums(outs):
+    """Per output: sum of the value bits as int64 (NaN payloads made one)."""
+    import torch
+    res = []
+    for v in outs.values():
+        bits = torch.where(torch.isnan(v), torch.full_like(v, float("nan")), v).view(torch.int32)
+        res.append(int(bits.to(torch.int64).sum().item()))
+    return res
+
+
+def run_jobs(args):
+    import torch
+    from kunquant_amd import autotune
+    from kunquant_amd.runner import KunRunner as kr
+    from kunquant_amd.workloads import catalog, jobs
+
+    specs = jobs.draw_jobs(args.jobs, seed=4)
+    mods = {g: catalog.load(g)[1] for g in jobs.JOB_GRAPHS}
+    ex = kr.createSingleThreadExecutor()
+    items = []
+    for sp in specs:
+        mod = mods[sp.graph]
+        d = jobs.ohlcv_device(sp.stocks, sp.days, sp.seed)
+        ins = {k: d[k] for k in mod.getInputNames()}
+        del d
+        outs = {k: torch.empty((sp.days, sp.stocks), dtype=torch.float32, device="cuda:0")
+                for k in mod.getOutputNames()}
+        items.append((sp, mod, ins, outs))
+    torch.cuda.synchronize()
+    tuples = [(mod, {k: v.data_ptr() for k, v in list(i.items()) + list(o.items())}, sp.stocks, sp.days, 0, sp.days)
+              for sp, mod, i, o in items]
+    cells = sum(len(o) * sp.stocks * sp.days for sp, _, _, o in items)
+    print(f"jobs: {len(items)} jobs, {cells:.3g} output cells", file=sys.stderr, flush=True)
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        ex.synchronize()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        ex.synchronize()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps * 1e3
+
+    # 1. sequential plain runs, device-resident, default chunking
+    def sequential():
+        for t in tuples:
+            kr.runGraphDevice(ex, *t)
+            ex.synchronize()
+    seq_ms = timed(sequential, args.steps, args.warmup)
+    ref_sums = [_checksums(o) for _, _, _, o in items]
+    ex.release_temps()
+    print(f"jobs: sequential device {seq_ms:.1f} ms", file=sys.stderr, flush=True)
+
+    # 2. autotune every job's chunking (kept on the executor per module and shape)
+    t0 = time.perf_counter()
+    picked = {}
+    for t in tuples:
+        best, _ = autotune.tune(ex, t[0], t[1], t[2], t[3], reps=1)
+        picked[best] = picked.get(best, 0) + 1
+        ex.release_temps()
+    tune_s = time.perf_counter() - t0
+    print(f"jobs: autotune {tune_s:.1f} s", file=sys.stderr, flush=True)
+
+    # 3. batched enqueue (one call, kernels of all jobs interleaved over the lanes)
+    for _, _, _, o in items:
+        for v in o.values():
+            v.fill_(0.0)
+    torch.cuda.synchronize()
+    batch_ms = timed(lambda: kr.runGraphBatchDevice(ex, tuples), args.steps, args.warmup)
+    same = all(_checksums(o) == r for (_, _, _, o), r in zip(items, ref_sums))
+    ex.release_temps()
+    print(f"jobs: batched {batch_ms:.1f} ms, results equal {same}", file=sys.stderr, flush=True)
+
+    # 5. sequential plain runGraph on host buffers (the reference's API: PCIe both ways)
+    host_ms = None
+    if not args.no_host_baseline:
+        host = [(mod, {k: v.cpu().numpy() for k, v in i.items()}, sp.days) for sp, mod, i, _ in items]
+        t0 = time.perf_counter()
+        for mod, hin, days in host:
+            kr.runGraph(ex, mod, hin, 0, days)
+        host_ms = (time.perf_counter() - t0) * 1e3
+        ex.release_temps()
+
+    result = {
+        "metric": "factor-cells/s",
+        "value": cells / (batch_ms / 1e3),
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": batch_ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded OHLCV generated on the device, 10% NaN cells)",
+        "config": {"workload": f"{len(items)} batched uneven jobs (stocks 50..6000, days 250..2500, "
+                               f"20..60-factor graphs), one batched enqueue, autotuned chunking",
+                   "jobs": len(items), "output_cells": cells,
+                   "graphs": {gname: sum(1 for sp in specs if sp.graph == gname) for gname in jobs.JOB_GRAPHS}},
+        "batched": {"batched_ms": batch_ms, "sequential_device_ms": seq_ms,
+                    "sequential_host_runGraph_ms": host_ms,
+                    "speedup_vs_sequential_device": seq_ms / batch_ms,
+                    "speedup_vs_sequential_host": host_ms / batch_ms if host_ms else None,
+                    "graph_capture": "switched off (kunCaptureGraphBatch crashed the process)",
+                    "autotune_s": tune_s,
+                    "policies": {f"{p[0]}/{p[1]}/{p[2]}": n for p, n in picked.items()},
+                    "results_equal_sequential": same},
+    }
     print(json.dumps(result), flush=True)
 
 

@@ -28,6 +28,7 @@ typedef void *KunLibraryHandle;
 typedef void *KunModuleHandle;
 typedef void *KunBufferNameMapHandle;
 typedef void *KunStreamContextHandle;
+typedef void *KunBatchGraphHandle;
 
 #define KUN_API_VERSION 1
 
@@ -127,6 +128,33 @@ KUN_API int kunRunGraphDevice(KunExecutorHandle exec, KunModuleHandle m, KunBuff
  * Results do not depend on it. Drops cached device temporaries. */
 KUN_API int kunExecutorSetChunking(KunExecutorHandle exec, long target_workgroups, long halo_factor,
                                   long min_rows);
+/* Frees the temporaries the executor keeps for device-resident runs (one
+ * set per module, shape and job position); waits for the device first.
+ * No counterpart in the reference, whose executor allocates per call. */
+KUN_API int kunExecutorReleaseTemps(KunExecutorHandle exec);
+/* Chunking for one (module, num_stocks, length) only, e.g. as picked by an
+ * autotuner; other shapes keep the executor's policy. */
+KUN_API int kunExecutorSetChunkingFor(KunExecutorHandle exec, KunModuleHandle m, size_t num_stocks,
+                                     size_t length, long target_workgroups, long halo_factor, long min_rows);
+/* Independent jobs on device-resident buffers enqueued together (job j:
+ * mods[j], bufs[j] holding device pointers, num_stocks[j], ...): their
+ * kernels interleave over the executor's lanes. Each job slot keeps its own
+ * temporaries. Returns without synchronising. */
+KUN_API int kunRunGraphBatchDevice(KunExecutorHandle exec, int njobs, const KunModuleHandle *mods,
+                                   const KunBufferNameMapHandle *bufs, const size_t *num_stocks,
+                                   const size_t *total_time, const size_t *cur_time, const size_t *length,
+                                   void *hip_stream);
+/* The same batch captured once as a HIP graph (with temporaries of its own)
+ * and replayed by kunLaunchGraphBatch: one launch for every kernel of every
+ * job. The buffers' device pointers are baked in. NULL on failure.
+ * Currently disabled: always returns NULL with an error message (capture
+ * crashed the calling process on MI355X; cause not found yet). */
+KUN_API KunBatchGraphHandle kunCaptureGraphBatch(KunExecutorHandle exec, int njobs, const KunModuleHandle *mods,
+                                                 const KunBufferNameMapHandle *bufs, const size_t *num_stocks,
+                                                 const size_t *total_time, const size_t *cur_time,
+                                                 const size_t *length);
+KUN_API int kunLaunchGraphBatch(KunBatchGraphHandle g, void *hip_stream);
+KUN_API void kunDestroyGraphBatch(KunBatchGraphHandle g);
 /* Blocks until the executor's stream is idle. */
 KUN_API int kunExecutorSynchronize(KunExecutorHandle exec);
 
@@ -160,6 +188,12 @@ KUN_API int kunModuleStageOutput(KunModuleHandle m, int stage, int j);
 KUN_API int kunModuleStageNumKernels(KunModuleHandle m, int stage);
 KUN_API int kunModuleKernelInfo(KunModuleHandle m, int stage, int kern, int *kind, int *ncomp, int *halo,
                                 int *lds_bytes, int *io_values);
+/* Device memory a batch run of `length` rows over `num_stocks` stocks holds
+ * besides its inputs and outputs (temporaries, values handed between a
+ * stage's kernels, HBM ring columns), under the default row chunking; the
+ * [rows][stocks] slots it reuses are counted in *num_slots (may be null).
+ * Returns 0 on error. Host-only: touches no GPU. */
+KUN_API size_t kunModuleDeviceBytes(KunModuleHandle m, size_t num_stocks, size_t length, int *num_slots);
 /* Kernel timing with HIP events recorded on the stream each kernel is
  * launched on. While enabled, every batch run records an event pair around
  * each kernel; kunExecutorStageTime synchronises and returns the accumulated

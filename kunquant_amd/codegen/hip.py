@@ -115,6 +115,8 @@ class _Lowering:
                 self.slot[b.idx] = len(self.slot)
         if len(self.slot) > MAX_STAGE_BUFS:
             raise RuntimeError(f"stage {stage.name} uses {len(self.slot)} buffers (limit {MAX_STAGE_BUFS})")
+        # the runtime passes inputs then outputs, so materialised values start at len(slot)
+        assert len(self.slot) == len(stage.in_bufs) + len(stage.out_bufs), stage.name
 
     def v(self, op) -> str:
         return f"v{self.idx[op]}"
@@ -678,6 +680,12 @@ class CompEmitter(_Lowering):
             return f"b{b.idx}[kun::at<{lay}>(({row}), s, A.S, A.L, A.blk)]"
         return f"m{k}[(size_t)({row}) * A.S + s]"
 
+    def mat_buf(self, op) -> int:
+        """Entry of kun::StageArgs::bufs holding materialised value `op`
+        ([L][S], placed by the runtime's memory plan): after the stage's
+        buffers."""
+        return len(self.slot) + self.plan.mat[op]
+
     def lag_load(self, op, row: str) -> str:
         """Row `row` of a row source, NaN before the run start; the load is
         unconditional (row clamped at 0) so the row loop has no branch."""
@@ -801,14 +809,13 @@ class CompEmitter(_Lowering):
             L.append(f"{T} *__restrict__ stT = ({T} *)((char *)A.scratch + A.zscr[e]);")
             L.append(f"int32_t *__restrict__ istp = (int32_t *)(stT + (size_t){self.n_t - self.lds_t} * cst);")
         L.extend(self.buffer_ptrs())
-        rows = "(size_t)A.L * A.S"
         for op in self.ext:
             k = self.idx[op]
-            L.append(f"const {T} *__restrict__ m{k} = (const {T} *)A.mat + (size_t){self.plan.mat[op]} * {rows};")
+            L.append(f"const {T} *__restrict__ m{k} = (const {T} *)A.bufs[{self.mat_buf(op)}];")
         for op in self.ops:
             if op in self.computed and op in self.plan.mat:
                 k = self.idx[op]
-                L.append(f"{T} *__restrict__ w{k} = ({T} *)A.mat + (size_t){self.plan.mat[op]} * {rows};")
+                L.append(f"{T} *__restrict__ w{k} = ({T} *)A.bufs[{self.mat_buf(op)}];")
         for op in sorted(self.single, key=lambda o: self.idx[o]):
             b = self.m.buffer(op.attrs["name"])
             L.append(f"const {T} v{self.idx[op]} = b{b.idx}[s];")
@@ -916,6 +923,9 @@ class LevelKernel:
                 elif op in c.computed and op in plan.mat:
                     io.add(("m", c.idx[op]))
         self.io_values = len(io)
+        if len(self.comps[0].slot) + len(plan.mat) > MAX_STAGE_BUFS:
+            raise RuntimeError(f"stage {stage.name}: {len(self.comps[0].slot)} buffers and {len(plan.mat)} "
+                               f"materialised values exceed {MAX_STAGE_BUFS} kernel buffer entries")
         # module buffers read / written, and materialised values produced
         self.reads = sorted({module.buffer(op.attrs["name"]).idx for c in self.comps for op in c.ops
                              if isinstance(op, Input)})
@@ -923,6 +933,8 @@ class LevelKernel:
                               if isinstance(op, Output)})
         self.produced = {op for c in self.comps for op in c.computed if op in plan.mat}
         self.consumed = {op for c in self.comps for op in c.ext}
+        self.mat_writes = sorted(plan.mat[op] for op in self.produced)
+        self.mat_reads = sorted(plan.mat[op] for op in self.consumed)
         self.kdeps: List[int] = []
         if self.lds_bytes > LDS_LIMIT:
             raise RuntimeError(f"stage {stage.name}: {self.lds_bytes} bytes of LDS exceed the gfx950 limit")
@@ -1016,7 +1028,8 @@ def emit_module(module: ModuleIR, lds_budget: int = None) -> str:
             for lk in kerns:
                 for tag, vals in (("cost", [c.cost for c in lk.comps]), ("halo", [c.halo for c in lk.comps]),
                                   ("spill", [c.spill_bytes for c in lk.comps]), ("reads", lk.reads),
-                                  ("writes", lk.writes), ("kdeps", lk.kdeps)):
+                                  ("writes", lk.writes), ("kdeps", lk.kdeps), ("mreads", lk.mat_reads),
+                                  ("mwrites", lk.mat_writes)):
                     body = ", ".join(str(v) for v in vals) if vals else "-1"
                     parts.append(f"static const int {lk.kname}_{tag}[] = {{{body}}};")
             parts.append(f"static const KunKernelDesc kun_{mid}_s{st.idx}_kerns[] = {{")
@@ -1024,7 +1037,8 @@ def emit_module(module: ModuleIR, lds_budget: int = None) -> str:
                 parts.append(f"  {{(const void *)&{lk.kname}, {lk.kind}, {len(lk.comps)}, {lk.halo}, "
                              f"{lk.lds_bytes}, {lk.io_values}, {lk.kname}_cost, {lk.kname}_halo, {lk.kname}_spill, "
                              f"{lk.kname}_reads, {len(lk.reads)}, {lk.kname}_writes, {len(lk.writes)}, "
-                             f"{lk.kname}_kdeps, {len(lk.kdeps)}}},")
+                             f"{lk.kname}_kdeps, {len(lk.kdeps)}, {lk.kname}_mreads, {len(lk.mat_reads)}, "
+                             f"{lk.kname}_mwrites, {len(lk.mat_writes)}}},")
             parts.append("};")
     if module.stream_mode:
         parts.append(f"static const void *kun_{mid}_inits[] = {{"

@@ -64,6 +64,91 @@ KUN_API int kunExecutorSetChunking(KunExecutorHandle exec, long target_workgroup
     });
 }
 
+KUN_API int kunExecutorReleaseTemps(KunExecutorHandle exec) {
+    return guarded([&] {
+        Executor &ex = **unwrap_exec(exec);
+        std::lock_guard<std::mutex> g(ex.mu);
+        HIPCHECK(hipSetDevice(ex.device));
+        HIPCHECK(hipDeviceSynchronize());
+        ex.free_temps();
+    });
+}
+
+KUN_API int kunExecutorSetChunkingFor(KunExecutorHandle exec, KunModuleHandle m, size_t num_stocks,
+                                     size_t length, long target_workgroups, long halo_factor, long min_rows) {
+    return guarded([&] {
+        if (target_workgroups < 1 || halo_factor < 0 || min_rows < 1) throw Error("bad chunking parameters");
+        Executor &ex = **unwrap_exec(exec);
+        std::lock_guard<std::mutex> g(ex.mu);
+        HIPCHECK(hipStreamSynchronize(ex.stream));
+        ChunkPolicy p;
+        p.target_workgroups = target_workgroups;
+        p.halo_factor = halo_factor;
+        p.min_rows = min_rows;
+        ex.tuned[std::make_tuple((const void *)m, num_stocks, length)] = p;
+        // drop cached temporaries of that module and shape: their layout changes
+        for (auto it = ex.temps.begin(); it != ex.temps.end();) {
+            if (std::get<0>(it->first) == (const void *)m && std::get<1>(it->first) == num_stocks &&
+                std::get<2>(it->first) == length) {
+                free_device_temps(it->second);
+                it = ex.temps.erase(it);
+            } else {
+                ++it;
+            }
+        }
+    });
+}
+
+static std::vector<BatchJob> make_jobs(int njobs, const KunModuleHandle *mods, const KunBufferNameMapHandle *bufs,
+                                       const size_t *num_stocks, const size_t *total_time, const size_t *cur_time,
+                                       const size_t *length) {
+    if (njobs < 1) throw Error("a batch needs at least one job");
+    std::vector<BatchJob> jobs(njobs);
+    for (int j = 0; j < njobs; j++)
+        jobs[j] = BatchJob{unwrap_mod(mods[j]), unwrap_map(bufs[j]), num_stocks[j], total_time[j], cur_time[j],
+                           length[j]};
+    return jobs;
+}
+
+KUN_API int kunRunGraphBatchDevice(KunExecutorHandle exec, int njobs, const KunModuleHandle *mods,
+                                   const KunBufferNameMapHandle *bufs, const size_t *num_stocks,
+                                   const size_t *total_time, const size_t *cur_time, const size_t *length,
+                                   void *hip_stream) {
+    return guarded([&] {
+        run_jobs_device(**unwrap_exec(exec), make_jobs(njobs, mods, bufs, num_stocks, total_time, cur_time, length),
+                        (hipStream_t)hip_stream);
+    });
+}
+
+KUN_API KunBatchGraphHandle kunCaptureGraphBatch(KunExecutorHandle exec, int njobs, const KunModuleHandle *mods,
+                                                 const KunBufferNameMapHandle *bufs, const size_t *num_stocks,
+                                                 const size_t *total_time, const size_t *cur_time,
+                                                 const size_t *length) {
+    CapturedBatch *cb = nullptr;
+    guarded([&] {
+        // Switched off: capturing a batch of jobs ended the calling process
+        // with a segmentation fault on MI355X (ROCm 7.2), before or during
+        // hipStreamBeginCapture..hipGraphInstantiate, cause not found yet.
+        // Batches run through kunRunGraphBatchDevice meanwhile.
+        (void)capture_jobs;
+        throw Error("kunCaptureGraphBatch is disabled in this build (batch graph capture crashed the process); "
+                    "use kunRunGraphBatchDevice");
+        cb = capture_jobs(**unwrap_exec(exec), make_jobs(njobs, mods, bufs, num_stocks, total_time, cur_time, length));
+    });
+    return cb;
+}
+
+KUN_API int kunLaunchGraphBatch(KunBatchGraphHandle g, void *hip_stream) {
+    return guarded([&] {
+        CapturedBatch *cb = (CapturedBatch *)g;
+        if (!cb) throw Error("null batch graph");
+        HIPCHECK(hipSetDevice(cb->device));
+        HIPCHECK(hipGraphLaunch(cb->exec, (hipStream_t)hip_stream));
+    });
+}
+
+KUN_API void kunDestroyGraphBatch(KunBatchGraphHandle g) { delete (CapturedBatch *)g; }
+
 KUN_API int kunExecutorSynchronize(KunExecutorHandle exec) {
     return guarded([&] { HIPCHECK(hipStreamSynchronize((*unwrap_exec(exec))->stream)); });
 }
@@ -153,6 +238,16 @@ KUN_API int kunModuleKernelInfo(KunModuleHandle m, int stage, int kern, int *kin
     *lds_bytes = kd.lds_bytes;
     *io_values = kd.io_values;
     return 0;
+}
+
+KUN_API size_t kunModuleDeviceBytes(KunModuleHandle m, size_t num_stocks, size_t length, int *num_slots) {
+    size_t bytes = 0;
+    int rc = guarded([&] {
+        BatchLayout lay = batch_layout(unwrap_mod(m), num_stocks, length, ChunkPolicy::from_env(), 4);
+        bytes = lay.bytes;
+        if (num_slots) *num_slots = lay.nslots;
+    });
+    return rc ? 0 : bytes;
 }
 
 KUN_API int kunExecutorEnableProfiling(KunExecutorHandle exec, int on) {

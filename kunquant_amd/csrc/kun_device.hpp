@@ -336,7 +336,8 @@ template <int LAYOUT> __device__ __forceinline__ size_t at(long row, long s, lon
 #endif
 
 // Arguments of a batch stage kernel, passed by value: the stage's buffers in
-// the order of its descriptor (inputs, then outputs).
+// the order of its descriptor (inputs, then outputs), then its materialised
+// values.
 #ifndef KUN_MAX_COMPS
 #define KUN_MAX_COMPS 64
 #endif
@@ -344,7 +345,6 @@ template <int LAYOUT> __device__ __forceinline__ size_t at(long row, long s, lon
 struct StageArgs {
     void *bufs[KUN_MAX_STAGE_BUFS];
     void *scratch;     // HBM columns of rings that do not fit in LDS, sorted windows
-    void *mat;         // [slot][L][S] values handed between the kernels of the stage
     long S;            // stocks
     long T;            // rows of the user inputs
     long start;        // first row of user inputs to compute

@@ -8,7 +8,7 @@
 #ifndef KUN_MODULE_ABI_H
 #define KUN_MODULE_ABI_H
 
-#define KUN_MODULE_ABI_VERSION 0x4B554E05u
+#define KUN_MODULE_ABI_VERSION 0x4B554E06u
 
 enum { KUN_BUF_INPUT = 0, KUN_BUF_OUTPUT = 1, KUN_BUF_TEMP = 2 };
 enum { KUN_STAGE_TS = 0, KUN_STAGE_RANK = 1, KUN_STAGE_SCALE = 2 };
@@ -45,6 +45,10 @@ typedef struct {
     int nwrites;
     const int *kdeps;      /* kernels of the same stage whose materialised values it reads */
     int nkdeps;
+    const int *mat_reads;  /* materialised values of the stage it reads / writes; value m is */
+    int nmat_reads;        /* passed as kun::StageArgs::bufs[nin + nout + m] */
+    const int *mat_writes;
+    int nmat_writes;
 } KunKernelDesc;
 
 typedef struct {
@@ -52,7 +56,7 @@ typedef struct {
     int kind;             /* KUN_STAGE_* */
     const KunKernelDesc *kerns; /* batch kernels of a TS stage, run in order */
     int nkern;
-    int nmat;             /* [rows][stocks] arrays handed between the stage's kernels */
+    int nmat;             /* [rows][stocks] values handed between the stage's kernels */
     const void *stream_kernel; /* one-tick kernel host stub (STREAM modules) */
     int block;            /* threads per workgroup */
     int state_bytes;      /* per-stock HBM bytes of carried state (STREAM modules) */

@@ -43,8 +43,11 @@ struct Pool {
     ~Pool();
 };
 
-// Scratch of a batch run: per stage, the arrays handed between its kernels;
-// per kernel, its row chunking and HBM columns.
+// Device memory of a batch run, one arena: per kernel, its row chunking and
+// HBM columns; then the [rows][stocks] slots that hold the module's
+// temporaries and the values handed between a stage's kernels (a slot is
+// reused once every kernel touching its previous value precedes the new
+// writer in the launch DAG).
 struct BatchLayout {
     struct Kern {
         size_t scratch = 0;       // arena offset of the kernel's scratch
@@ -54,8 +57,11 @@ struct BatchLayout {
             znch[KUN_MAX_COMPS] = {};
         long zscr[KUN_MAX_COMPS] = {};
     };
-    std::vector<size_t> stage_mat;
     std::vector<std::vector<Kern>> kern;
+    std::vector<long> buf_off;                // per module buffer: arena offset of a temporary, else -1
+    std::vector<std::vector<size_t>> mat_off; // per stage, per materialised value: arena offset
+    size_t slot_bytes = 0;
+    int nslots = 0;
     size_t bytes = 0;
     // launch DAG: one node per stage kernel, in a topological order
     struct Node {
@@ -76,7 +82,7 @@ struct ChunkPolicy {
 BatchLayout batch_layout(const KunModuleDesc *m, size_t S, size_t L, const ChunkPolicy &pol, int lanes);
 
 struct DeviceTemps {
-    std::vector<void *> bufs;
+    std::vector<void *> bufs; // per module buffer: the temporary's address in the arena, else null
     char *arena = nullptr;
     BatchLayout lay;
 };
@@ -95,7 +101,11 @@ struct Executor {
     std::mutex mu;
     ChunkPolicy chunking;
     // temporaries of device-resident runs, per (module, stocks, rows)
-    std::map<std::tuple<const void *, size_t, size_t>, DeviceTemps> temps;
+    // (module, stocks, rows, job slot); slot 0: single runs, j + 1: job j of a batch
+    std::map<std::tuple<const void *, size_t, size_t, int>, DeviceTemps> temps;
+    // chunking chosen for one (module, stocks, rows), e.g. by the autotuner
+    std::map<std::tuple<const void *, size_t, size_t>, ChunkPolicy> tuned;
+    ChunkPolicy policy_for(const KunModuleDesc *m, size_t S, size_t L) const;
     // per-stage kernel timing (bench / diagnostics)
     bool profiling = false;
     struct Pending {
@@ -130,6 +140,26 @@ void run_graph_host(Executor &ex, const KunModuleDesc *m, const NameMap &bufs, s
                     size_t start, size_t L);
 void run_graph_device(Executor &ex, const KunModuleDesc *m, const NameMap &bufs, size_t S, size_t T,
                       size_t start, size_t L, hipStream_t st);
+
+// one job of a batched device run
+struct BatchJob {
+    const KunModuleDesc *m;
+    const NameMap *bufs;
+    size_t S, T, start, L;
+};
+void run_jobs_device(Executor &ex, const std::vector<BatchJob> &jobs, hipStream_t st);
+
+// a batch of jobs captured as a HIP graph, with temporaries of its own
+struct CapturedBatch {
+    int device = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<DeviceTemps> temps;
+    ~CapturedBatch();
+};
+CapturedBatch *capture_jobs(Executor &ex, const std::vector<BatchJob> &jobs);
+DeviceTemps make_temps(Executor &ex, const KunModuleDesc *m, size_t S, size_t T, size_t L);
+void free_device_temps(DeviceTemps &t);
 
 // ---- stream contexts ----------------------------------------------------
 struct StreamCtx;

@@ -285,19 +285,106 @@ static void plan_kernel(const KunKernelDesc &kd, size_t S, size_t L, const Chunk
     k.zfirst[nc] = (int)k.blocks;
 }
 
+// Slots of the [rows][stocks] values of a batch run (module temporaries and
+// materialised values). Nodes are visited in launch order; a value written
+// by node n takes a slot whose previous value was written and read only by
+// nodes that finish before n starts: ancestors of n through the DAG's
+// edges and through the order of each lane (a lane is one HIP stream; an
+// executor with fewer streams folds lanes together, which keeps each
+// lane's order). Slots are added when none is free.
+static void plan_slots(const KunModuleDesc *m, size_t S, size_t L, size_t base, BatchLayout &lay) {
+    const size_t N = lay.nodes.size();
+    const size_t W = (N + 63) / 64;
+    std::vector<uint64_t> anc(N * W, 0);
+    std::map<int, int> last_on_lane;
+    for (size_t n = 0; n < N; n++) {
+        std::vector<int> pred = lay.nodes[n].deps;
+        auto it = last_on_lane.find(lay.nodes[n].lane);
+        if (it != last_on_lane.end()) pred.push_back(it->second);
+        last_on_lane[lay.nodes[n].lane] = (int)n;
+        for (int d : pred) {
+            for (size_t w = 0; w < W; w++) anc[n * W + w] |= anc[(size_t)d * W + w];
+            anc[n * W + (size_t)d / 64] |= 1ull << (d % 64);
+        }
+    }
+    auto is_anc = [&](size_t n, int a) { return (anc[n * W + (size_t)a / 64] >> (a % 64)) & 1; };
+    // value ids: module buffer b, or nbuf + first_mat[stage] + m
+    std::vector<int> first_mat(m->nstage + 1, 0);
+    for (int i = 0; i < m->nstage; i++)
+        first_mat[i + 1] = first_mat[i] + (m->stages[i].kind == KUN_STAGE_TS ? m->stages[i].nmat : 0);
+    const int nval = m->nbuf + first_mat[m->nstage];
+    std::vector<std::vector<int>> touch(nval); // nodes writing or reading each value
+    std::vector<std::vector<int>> writes(N);
+    auto is_temp = [&](int b) { return m->bufs[b].kind == KUN_BUF_TEMP; };
+    for (size_t n = 0; n < N; n++) {
+        const auto &nd = lay.nodes[n];
+        const KunStageDesc &sd = m->stages[nd.stage];
+        auto rd = [&](int v) { touch[v].push_back((int)n); };
+        auto wr = [&](int v) {
+            touch[v].push_back((int)n);
+            writes[n].push_back(v);
+        };
+        if (sd.kind == KUN_STAGE_TS) {
+            const KunKernelDesc &kd = sd.kerns[nd.kern];
+            for (int r = 0; r < kd.nreads; r++)
+                if (is_temp(kd.reads[r])) rd(kd.reads[r]);
+            for (int w = 0; w < kd.nwrites; w++)
+                if (is_temp(kd.writes[w])) wr(kd.writes[w]);
+            const int mb = m->nbuf + first_mat[nd.stage];
+            for (int r = 0; r < kd.nmat_reads; r++) rd(mb + kd.mat_reads[r]);
+            for (int w = 0; w < kd.nmat_writes; w++) wr(mb + kd.mat_writes[w]);
+        } else {
+            if (is_temp(sd.in[0])) rd(sd.in[0]);
+            if (is_temp(sd.out[0])) wr(sd.out[0]);
+        }
+    }
+    std::vector<int> slot_of(nval, -1), occupant;
+    auto place = [&](size_t n, int v) {
+        if (slot_of[v] >= 0) return;
+        for (size_t s = 0; s < occupant.size(); s++) {
+            bool free_ = true;
+            for (int t : touch[occupant[s]])
+                if (!is_anc(n, t)) {
+                    free_ = false;
+                    break;
+                }
+            if (free_) {
+                occupant[s] = v;
+                slot_of[v] = (int)s;
+                return;
+            }
+        }
+        slot_of[v] = (int)occupant.size();
+        occupant.push_back(v);
+    };
+    for (size_t n = 0; n < N; n++)
+        for (int v : writes[n]) place(n, v);
+    // values nothing writes (unused temporaries) keep slots of their own
+    for (int v = 0; v < nval; v++)
+        if (slot_of[v] < 0 && (v >= m->nbuf || is_temp(v))) {
+            slot_of[v] = (int)occupant.size();
+            occupant.push_back(v);
+        }
+    lay.slot_bytes = align256(std::max<size_t>(1, S * L) * elem_size(m));
+    lay.nslots = (int)occupant.size();
+    auto off = [&](int v) { return base + (size_t)slot_of[v] * lay.slot_bytes; };
+    lay.buf_off.assign(m->nbuf, -1);
+    for (int b = 0; b < m->nbuf; b++)
+        if (is_temp(b)) lay.buf_off[b] = (long)off(b);
+    lay.mat_off.assign(m->nstage, {});
+    for (int i = 0; i < m->nstage; i++)
+        for (int j = first_mat[i]; j < first_mat[i + 1]; j++) lay.mat_off[i].push_back(off(m->nbuf + j));
+    lay.bytes = base + (size_t)lay.nslots * lay.slot_bytes;
+}
+
 BatchLayout batch_layout(const KunModuleDesc *m, size_t S, size_t L, const ChunkPolicy &pol, int lanes) {
     BatchLayout lay;
-    const size_t esz = elem_size(m);
-    lay.stage_mat.assign(m->nstage, 0);
     lay.kern.resize(m->nstage);
     size_t off = 0;
     for (int i = 0; i < m->nstage; i++) {
         const KunStageDesc &sd = m->stages[i];
         if (sd.kind != KUN_STAGE_TS) continue;
-        if (sd.nmat) {
-            lay.stage_mat[i] = off;
-            off += align256((size_t)sd.nmat * L * S * esz);
-        }
+        if (sd.nin + sd.nout + sd.nmat > 256) throw Error("stage with too many buffers");
         for (int j = 0; j < sd.nkern; j++) {
             if (sd.kerns[j].ncomp > KUN_MAX_COMPS) throw Error("stage kernel with too many components");
             BatchLayout::Kern k;
@@ -306,8 +393,8 @@ BatchLayout batch_layout(const KunModuleDesc *m, size_t S, size_t L, const Chunk
             lay.kern[i].push_back(k);
         }
     }
-    lay.bytes = off;
     plan_nodes(m, S, L, lanes, lay);
+    plan_slots(m, S, L, align256(off), lay);
     return lay;
 }
 
@@ -330,13 +417,13 @@ static void enqueue_node(const KunModuleDesc *m, int si, int kj, const std::vect
         int n = 0;
         for (int j = 0; j < sd.nin; j++) a.bufs[n++] = dev[sd.in[j]];
         for (int j = 0; j < sd.nout; j++) a.bufs[n++] = dev[sd.out[j]];
+        for (int j = 0; j < sd.nmat; j++) a.bufs[n++] = arena + lay.mat_off[si][j];
         a.S = (long)S;
         a.T = (long)T;
         a.start = (long)start;
         a.L = (long)L;
         a.blk = m->blocking;
         a.scratch = arena + kl.scratch;
-        a.mat = sd.nmat ? arena + lay.stage_mat[si] : nullptr;
         a.nz = kl.nz;
         memcpy(a.zcomp, kl.zcomp, sizeof a.zcomp);
         memcpy(a.zfirst, kl.zfirst, sizeof a.zfirst);
@@ -484,25 +571,23 @@ void run_graph_host(Executor &ex, const KunModuleDesc *m, const NameMap &bufs, s
     const size_t esz = elem_size(m);
     std::vector<void *> dev(m->nbuf, nullptr);
     std::vector<size_t> bytes(m->nbuf, 0);
-    struct Release {
+    struct Release { // pooled inputs and outputs (temporaries live in the arena)
         Executor &ex;
+        const KunModuleDesc *m;
         std::vector<void *> &dev;
         std::vector<size_t> &bytes;
         ~Release() {
             hipStreamSynchronize(ex.stream);
-            for (size_t i = 0; i < dev.size(); i++) ex.pool.put(dev[i], bytes[i]);
+            for (size_t i = 0; i < dev.size(); i++)
+                if (m->bufs[i].kind != KUN_BUF_TEMP) ex.pool.put(dev[i], bytes[i]);
         }
-    } release{ex, dev, bytes};
+    } release{ex, m, dev, bytes};
     for (int b = 0; b < m->nbuf; b++) {
         const KunBufferDesc &d = m->bufs[b];
-        bytes[b] = buffer_elems(m, b, S, T, L) * esz;
         if (d.kind != KUN_BUF_TEMP && !bufs.count(d.name))
             throw Error(std::string("Buffer name not found: ") + d.name);
-        dev[b] = ex.pool.get(bytes[b]);
-        if (d.kind == KUN_BUF_INPUT)
-            HIPCHECK(hipMemcpyAsync(dev[b], bufs.at(d.name), bytes[b], hipMemcpyHostToDevice, ex.stream));
     }
-    BatchLayout lay = batch_layout(m, S, L, ex.chunking, (int)ex.lanes.size());
+    BatchLayout lay = batch_layout(m, S, L, ex.policy_for(m, S, L), (int)ex.lanes.size());
     struct ReleaseArena {
         Executor &ex;
         char *p = nullptr;
@@ -513,6 +598,17 @@ void run_graph_host(Executor &ex, const KunModuleDesc *m, const NameMap &bufs, s
         }
     } arena{ex, nullptr, lay.bytes};
     if (lay.bytes) arena.p = (char *)ex.pool.get(lay.bytes);
+    for (int b = 0; b < m->nbuf; b++) {
+        const KunBufferDesc &d = m->bufs[b];
+        if (d.kind == KUN_BUF_TEMP) {
+            dev[b] = arena.p + lay.buf_off[b];
+            continue;
+        }
+        bytes[b] = buffer_elems(m, b, S, T, L) * esz;
+        dev[b] = ex.pool.get(bytes[b]);
+        if (d.kind == KUN_BUF_INPUT)
+            HIPCHECK(hipMemcpyAsync(dev[b], bufs.at(d.name), bytes[b], hipMemcpyHostToDevice, ex.stream));
+    }
     enqueue_batch(ex, m, dev, lay, arena.p, S, T, start, L, ex.stream);
     for (int b = 0; b < m->nbuf; b++) {
         const KunBufferDesc &d = m->bufs[b];
@@ -523,29 +619,38 @@ void run_graph_host(Executor &ex, const KunModuleDesc *m, const NameMap &bufs, s
 }
 
 // Device-buffer run: temporaries from the pool (kept for reuse per shape).
-void run_graph_device(Executor &ex, const KunModuleDesc *m, const NameMap &bufs, size_t S, size_t T,
-                      size_t start, size_t L, hipStream_t st) {
-    check_batch_module(m, S, T, start, L);
-    std::lock_guard<std::mutex> g(ex.mu);
-    HIPCHECK(hipSetDevice(ex.device));
-    if (!st) st = ex.stream;
-    const size_t esz = elem_size(m);
-    auto key = std::make_tuple((const void *)m, S, L);
+ChunkPolicy Executor::policy_for(const KunModuleDesc *m, size_t S, size_t L) const {
+    auto it = tuned.find(std::make_tuple((const void *)m, S, L));
+    return it == tuned.end() ? chunking : it->second;
+}
+
+DeviceTemps make_temps(Executor &ex, const KunModuleDesc *m, size_t S, size_t T, size_t L) {
+    (void)T;
+    DeviceTemps t;
+    t.lay = batch_layout(m, S, L, ex.policy_for(m, S, L), (int)ex.lanes.size());
+    t.bufs.assign(m->nbuf, nullptr);
+    if (t.lay.bytes) HIPCHECK(hipMalloc((void **)&t.arena, t.lay.bytes));
+    for (int b = 0; b < m->nbuf; b++)
+        if (m->bufs[b].kind == KUN_BUF_TEMP) t.bufs[b] = t.arena + t.lay.buf_off[b];
+    return t;
+}
+
+void free_device_temps(DeviceTemps &t) {
+    if (t.arena) hipFree(t.arena);
+    t.bufs.clear();
+    t.arena = nullptr;
+}
+
+// temporaries of a device-resident run, cached per (module, shape, job slot)
+static const DeviceTemps &cached_temps(Executor &ex, const KunModuleDesc *m, size_t S, size_t T, size_t L,
+                                       int slot) {
+    auto key = std::make_tuple((const void *)m, S, L, slot);
     auto it = ex.temps.find(key);
-    if (it == ex.temps.end()) {
-        // temporaries first, then the scratch arena
-        DeviceTemps t;
-        t.lay = batch_layout(m, S, L, ex.chunking, (int)ex.lanes.size());
-        t.bufs.assign(m->nbuf, nullptr);
-        for (int b = 0; b < m->nbuf; b++)
-            if (m->bufs[b].kind == KUN_BUF_TEMP) {
-                size_t nb = buffer_elems(m, b, S, T, L) * esz;
-                HIPCHECK(hipMalloc(&t.bufs[b], nb ? nb : 256));
-            }
-        if (t.lay.bytes) HIPCHECK(hipMalloc((void **)&t.arena, t.lay.bytes));
-        it = ex.temps.emplace(key, std::move(t)).first;
-    }
-    const DeviceTemps &temps = it->second;
+    if (it == ex.temps.end()) it = ex.temps.emplace(key, make_temps(ex, m, S, T, L)).first;
+    return it->second;
+}
+
+static std::vector<void *> device_pointers(const KunModuleDesc *m, const NameMap &bufs, const DeviceTemps &temps) {
     std::vector<void *> dev(m->nbuf, nullptr);
     for (int b = 0; b < m->nbuf; b++) {
         const KunBufferDesc &d = m->bufs[b];
@@ -557,15 +662,139 @@ void run_graph_device(Executor &ex, const KunModuleDesc *m, const NameMap &bufs,
             dev[b] = f->second;
         }
     }
-    enqueue_batch(ex, m, dev, temps.lay, temps.arena, S, T, start, L, st);
+    return dev;
+}
+
+void run_graph_device(Executor &ex, const KunModuleDesc *m, const NameMap &bufs, size_t S, size_t T,
+                      size_t start, size_t L, hipStream_t st) {
+    check_batch_module(m, S, T, start, L);
+    std::lock_guard<std::mutex> g(ex.mu);
+    HIPCHECK(hipSetDevice(ex.device));
+    if (!st) st = ex.stream;
+    const DeviceTemps &temps = cached_temps(ex, m, S, T, L, 0);
+    enqueue_batch(ex, m, device_pointers(m, bufs, temps), temps.lay, temps.arena, S, T, start, L, st);
+}
+
+// Several independent jobs enqueued together: their launch DAGs interleave
+// over the lanes (job j shifts its lanes by j), so small jobs share the GPU.
+static void enqueue_jobs(Executor &ex, const std::vector<BatchJob> &jobs, const std::vector<const DeviceTemps *> &temps,
+                         hipStream_t st, bool profile) {
+    const int nl = (int)ex.lanes.size();
+    Executor *prof = profile && ex.profiling ? &ex : nullptr;
+    std::vector<std::vector<void *>> dev(jobs.size());
+    std::vector<size_t> first(jobs.size() + 1, 0);
+    for (size_t j = 0; j < jobs.size(); j++) {
+        dev[j] = device_pointers(jobs[j].m, *jobs[j].bufs, *temps[j]);
+        first[j + 1] = first[j] + temps[j]->lay.nodes.size();
+    }
+    if (nl == 0) {
+        for (size_t j = 0; j < jobs.size(); j++)
+            for (const auto &nd : temps[j]->lay.nodes)
+                enqueue_node(jobs[j].m, nd.stage, nd.kern, dev[j], temps[j]->lay, temps[j]->arena, jobs[j].S,
+                             jobs[j].T, jobs[j].start, jobs[j].L, st, prof);
+        return;
+    }
+    ex.ensure_stage_events((int)first.back());
+    HIPCHECK(hipEventRecord(ex.fork, st));
+    for (int l = 0; l < nl; l++) HIPCHECK(hipStreamWaitEvent(ex.lanes[l], ex.fork, 0));
+    std::vector<size_t> next(jobs.size(), 0);
+    bool more = true;
+    while (more) {
+        more = false;
+        for (size_t j = 0; j < jobs.size(); j++) {
+            const auto &nodes = temps[j]->lay.nodes;
+            if (next[j] >= nodes.size()) continue;
+            more = true;
+            const size_t n = next[j]++;
+            const auto &nd = nodes[n];
+            const int lane = (int)((nd.lane + j) % nl);
+            hipStream_t ls = ex.lanes[lane];
+            for (int d : nd.deps)
+                if ((int)((nodes[d].lane + j) % nl) != lane)
+                    HIPCHECK(hipStreamWaitEvent(ls, ex.stage_done[first[j] + d], 0));
+            enqueue_node(jobs[j].m, nd.stage, nd.kern, dev[j], temps[j]->lay, temps[j]->arena, jobs[j].S, jobs[j].T,
+                         jobs[j].start, jobs[j].L, ls, prof);
+            HIPCHECK(hipEventRecord(ex.stage_done[first[j] + n], ls));
+        }
+    }
+    for (int l = 0; l < nl; l++) {
+        HIPCHECK(hipEventRecord(ex.lane_done[l], ex.lanes[l]));
+        HIPCHECK(hipStreamWaitEvent(st, ex.lane_done[l], 0));
+    }
+}
+
+static void check_jobs(const std::vector<BatchJob> &jobs) {
+    for (const auto &jb : jobs) check_batch_module(jb.m, jb.S, jb.T, jb.start, jb.L);
+}
+
+void run_jobs_device(Executor &ex, const std::vector<BatchJob> &jobs, hipStream_t st) {
+    check_jobs(jobs);
+    std::lock_guard<std::mutex> g(ex.mu);
+    HIPCHECK(hipSetDevice(ex.device));
+    if (!st) st = ex.stream;
+    std::vector<const DeviceTemps *> temps;
+    for (size_t j = 0; j < jobs.size(); j++)
+        temps.push_back(&cached_temps(ex, jobs[j].m, jobs[j].S, jobs[j].T, jobs[j].L, (int)j + 1));
+    enqueue_jobs(ex, jobs, temps, st, true);
+}
+
+// KUN_TRACE_CAPTURE=1: report each phase of a batch capture on stderr
+static void capture_phase(const char *what, size_t n) {
+    static const bool on = env_long("KUN_TRACE_CAPTURE", 0) != 0;
+    if (on) {
+        fprintf(stderr, "[kun capture] %s (%zu)\n", what, n);
+        fflush(stderr);
+    }
+}
+
+CapturedBatch *capture_jobs(Executor &ex, const std::vector<BatchJob> &jobs) {
+    check_jobs(jobs);
+    std::lock_guard<std::mutex> g(ex.mu);
+    HIPCHECK(hipSetDevice(ex.device));
+    std::unique_ptr<CapturedBatch> cb(new CapturedBatch());
+    cb->device = ex.device;
+    cb->temps.reserve(jobs.size());
+    std::vector<const DeviceTemps *> temps;
+    for (const auto &jb : jobs) {
+        cb->temps.push_back(make_temps(ex, jb.m, jb.S, jb.T, jb.L));
+        temps.push_back(&cb->temps.back());
+    }
+    hipStream_t st = ex.stream;
+    HIPCHECK(hipStreamSynchronize(st));
+    capture_phase("temporaries allocated, begin", jobs.size());
+    HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+        enqueue_jobs(ex, jobs, temps, st, false);
+        capture_phase("enqueued", ex.stage_done.size());
+    } catch (...) {
+        hipGraph_t broken = nullptr;
+        hipStreamEndCapture(st, &broken);
+        if (broken) hipGraphDestroy(broken);
+        throw;
+    }
+    HIPCHECK(hipStreamEndCapture(st, &cb->graph));
+    size_t nnodes = 0;
+    HIPCHECK(hipGraphGetNodes(cb->graph, nullptr, &nnodes));
+    capture_phase("captured, graph nodes", nnodes);
+    // hipGraphInstantiate writes the failing node and a log through its
+    // out-parameters when instantiation fails: give it both
+    hipGraphNode_t bad = nullptr;
+    char log[1024] = {0};
+    hipError_t e = hipGraphInstantiate(&cb->exec, cb->graph, &bad, log, sizeof log - 1);
+    if (e != hipSuccess)
+        throw Error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e) + (log[0] ? ": " : "") + log);
+    capture_phase("instantiated", nnodes);
+    return cb.release();
+}
+
+CapturedBatch::~CapturedBatch() {
+    if (exec) hipGraphExecDestroy(exec);
+    if (graph) hipGraphDestroy(graph);
+    for (auto &t : temps) free_device_temps(t);
 }
 
 void Executor::free_temps() {
-    for (auto &kv : temps) {
-        for (void *p : kv.second.bufs)
-            if (p) hipFree(p);
-        if (kv.second.arena) hipFree(kv.second.arena);
-    }
+    for (auto &kv : temps) free_device_temps(kv.second);
     temps.clear();
 }
 

@@ -64,9 +64,17 @@ def _load():
     sig("kunExecutorStageTime", i, vp, vp, i, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long))
     sig("kunExecutorKernelTime", i, vp, vp, i, i, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long))
     sig("kunExecutorSetChunking", i, vp, ctypes.c_long, ctypes.c_long, ctypes.c_long)
+    sig("kunExecutorReleaseTemps", i, vp)
+    sig("kunExecutorSetChunkingFor", i, vp, vp, sz, sz, ctypes.c_long, ctypes.c_long, ctypes.c_long)
+    szp = ctypes.POINTER(sz)
+    sig("kunRunGraphBatchDevice", i, vp, i, ctypes.POINTER(vp), ctypes.POINTER(vp), szp, szp, szp, szp, vp)
+    sig("kunCaptureGraphBatch", vp, vp, i, ctypes.POINTER(vp), ctypes.POINTER(vp), szp, szp, szp, szp)
+    sig("kunLaunchGraphBatch", i, vp, vp)
+    sig("kunDestroyGraphBatch", None, vp)
     sig("kunModuleStageNumKernels", i, vp, i)
     ip = ctypes.POINTER(ctypes.c_int)
     sig("kunModuleKernelInfo", i, vp, i, i, ip, ip, ip, ip, ip)
+    sig("kunModuleDeviceBytes", sz, vp, sz, sz, ip)
     sig("kunCreateStreamEx", i, vp, vp, sz, vp, ctypes.POINTER(vp))
     sig("kunStreamSerializeStates", i, vp, i, vp, ctypes.POINTER(sz))
     sig("kunQueryBufferHandle", sz, vp, cp)
@@ -121,6 +129,19 @@ class Executor:
         if _load().kunExecutorKernelTime(self._h, mod._h, stage, kern, ctypes.byref(ms), ctypes.byref(n)):
             raise RuntimeError(_err())
         return ms.value, n.value
+
+    def set_chunking_for(self, mod: "Module", num_stocks: int, length: int, target_workgroups: int,
+                         halo_factor: int, min_rows: int):
+        """Chunking for one (module, stocks, rows) only (see autotune)."""
+        if _load().kunExecutorSetChunkingFor(self._h, mod._h, num_stocks, length, target_workgroups, halo_factor,
+                                             min_rows):
+            raise RuntimeError(_err())
+
+    def release_temps(self):
+        """Free the temporaries kept for device-resident runs (waits for the
+        device)."""
+        if _load().kunExecutorReleaseTemps(self._h):
+            raise RuntimeError(_err())
 
     def set_chunking(self, target_workgroups: int = 32768, halo_factor: int = 3, min_rows: int = 64):
         """How row-chunked stage kernels split the rows (results do not
@@ -203,6 +224,16 @@ class Module:
                 raise RuntimeError(_err())
             res.append(tuple(x.value for x in v))
         return res
+
+    def device_bytes(self, num_stocks: int, length: int):
+        """(bytes, slots): device memory a batch run holds besides its
+        inputs and outputs, and the [rows][stocks] slots among them."""
+        L = _load()
+        n = ctypes.c_int()
+        b = L.kunModuleDeviceBytes(self._h, num_stocks, length, ctypes.byref(n))
+        if b == 0 and _err():
+            raise RuntimeError(_err())
+        return b, n.value
 
 
 class Library:
@@ -363,6 +394,59 @@ def runGraphDevice(exec: Executor, mod: Module, buffers: Dict[str, int], num_sto
         L.kunDestoryBufferNameMap(h)
     if rc:
         raise RuntimeError(_err())
+
+
+def _job_arrays(jobs):
+    """jobs: [(module, {name: device pointer}, num_stocks, total_time, cur_time, length)]"""
+    L = _load()
+    n = len(jobs)
+    maps = [_name_map(j[1]) for j in jobs]
+    mods = (ctypes.c_void_p * n)(*[j[0]._h for j in jobs])
+    bufs = (ctypes.c_void_p * n)(*maps)
+    cols = [(ctypes.c_size_t * n)(*[int(j[k]) for j in jobs]) for k in range(2, 6)]
+    return maps, mods, bufs, cols
+
+
+def runGraphBatchDevice(exec: Executor, jobs, stream: int = 0):
+    """Independent device-resident jobs enqueued together; each job is
+    (module, {name: device pointer}, num_stocks, total_time, cur_time,
+    length). Returns without synchronising."""
+    L = _load()
+    maps, mods, bufs, cols = _job_arrays(jobs)
+    try:
+        rc = L.kunRunGraphBatchDevice(exec._h, len(jobs), mods, bufs, *cols,
+                                      ctypes.c_void_p(stream) if stream else None)
+    finally:
+        for h in maps:
+            L.kunDestoryBufferNameMap(h)
+    if rc:
+        raise RuntimeError(_err())
+
+
+class BatchGraph:
+    """A batch of jobs captured once as a HIP graph and replayed with one
+    launch (device pointers are fixed at capture)."""
+
+    def __init__(self, exec: Executor, jobs):
+        L = _load()
+        self._keep = [exec] + [j[0] for j in jobs]
+        maps, mods, bufs, cols = _job_arrays(jobs)
+        try:
+            self._h = L.kunCaptureGraphBatch(exec._h, len(jobs), mods, bufs, *cols)
+        finally:
+            for h in maps:
+                L.kunDestoryBufferNameMap(h)
+        if not self._h:
+            raise RuntimeError(_err())
+
+    def launch(self, stream: int = 0):
+        if _load().kunLaunchGraphBatch(self._h, ctypes.c_void_p(stream) if stream else None):
+            raise RuntimeError(_err())
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.kunDestroyGraphBatch(self._h)
+            self._h = None
 
 
 class StreamContext:

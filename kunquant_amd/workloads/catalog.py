@@ -26,6 +26,11 @@ CATALOG: Dict[str, Tuple[Callable, dict]] = {
                                                   options={"no_fast_stat": True})),
     "aligned_f32": (aligned_graph, dict(input_layout="TS", output_layout="TS", allow_unaligned=False)),
     "cs_f32": (cross_section_graph, dict(input_layout="TS", output_layout="TS")),
+    # graphs of the batched-jobs workload (64 uneven jobs draw from this pool)
+    "jobs_g0": (lambda: random_graph(20, 4000), dict(input_layout="TS", output_layout="TS")),
+    "jobs_g1": (lambda: random_graph(30, 4001), dict(input_layout="TS", output_layout="TS")),
+    "jobs_g2": (lambda: random_graph(45, 4002), dict(input_layout="TS", output_layout="TS")),
+    "jobs_g3": (lambda: random_graph(60, 4003), dict(input_layout="TS", output_layout="TS")),
     "cs_f64": (cross_section_graph, dict(dtype="double", input_layout="TS", output_layout="TS",
                                         options={"no_fast_stat": "no_warn"})),
     "stream_f64": (stream_graph, dict(dtype="double", input_layout="STREAM", output_layout="STREAM")),
@@ -55,9 +60,18 @@ def load(name: str):
 
 
 def build_all(jobs: int = 8, names=None):
+    """Compile every catalog module (several hipcc processes at a time)."""
+    from concurrent.futures import ThreadPoolExecutor
     from ..build import compile_module_sources
     from ..codegen.hip import emit_module
     os.makedirs(MODULE_DIR, exist_ok=True)
-    for name in (names or CATALOG):
-        src = emit_module(module_ir(name))
-        compile_module_sources([src], library_path(name), os.path.join(MODULE_DIR, "_src", name), jobs=jobs)
+    todo = list(names or CATALOG)
+    srcs = {name: emit_module(module_ir(name)) for name in todo}
+
+    def one(name):
+        return compile_module_sources([srcs[name]], library_path(name), os.path.join(MODULE_DIR, "_src", name),
+                                      jobs=1)
+
+    # largest first, so the long compiles overlap the short ones
+    with ThreadPoolExecutor(max_workers=max(1, jobs)) as pool:
+        list(pool.map(one, sorted(todo, key=lambda n: -len(srcs[n]))))

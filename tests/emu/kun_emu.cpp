@@ -35,12 +35,12 @@ extern "C" int kun_emu_run_stage(const KunModuleDesc *m, int stage, void **bufs,
         int n = 0;
         for (int j = 0; j < sd.nin; j++) a.bufs[n++] = bufs[sd.in[j]];
         for (int j = 0; j < sd.nout; j++) a.bufs[n++] = bufs[sd.out[j]];
+        for (int j = 0; j < sd.nmat; j++) a.bufs[n++] = mat.data() + (size_t)j * L * S * esz;
         a.S = S;
         a.T = T;
         a.start = start;
         a.L = L;
         a.blk = m->blocking;
-        a.mat = mat.data();
         a.nz = kd.ncomp;
         size_t off = 0;
         long blocks = 0;

@@ -52,3 +52,18 @@ def test_library_errors():
     lib, _ = catalog.load("simple_f32")
     with pytest.raises(RuntimeError, match="Module name not found"):
         lib.getModule("testRuntimeModule2")
+
+
+def test_device_bytes_reuse_slots():
+    """The memory plan of a batch run (host-side, no GPU): module temporaries
+    and values handed between a stage's kernels share fewer [rows][stocks]
+    slots than there are temporaries; a module without temporaries still
+    gets slots for its materialised values only."""
+    _, mod = catalog.load("cfg2_f32")
+    nbytes, slots = mod.device_bytes(8192, 1024)
+    ntemp = sum(1 for _, kind, _, _ in mod.buffers if kind == 2)
+    assert 0 < slots < ntemp
+    assert nbytes >= slots * 8192 * 1024 * 4
+    _, small = catalog.load("simple_f32")
+    nb2, slots2 = small.device_bytes(100, 50)
+    assert slots2 <= 4 and nb2 >= slots2 * 100 * 50 * 4

@@ -28,6 +28,7 @@ typedef void *KunLibraryHandle;
 typedef void *KunModuleHandle;
 typedef void *KunBufferNameMapHandle;
 typedef void *KunStreamContextHandle;
+typedef void *KunStreamPoolHandle;
 typedef void *KunBatchGraphHandle;
 
 #define KUN_API_VERSION 1
@@ -157,6 +158,41 @@ KUN_API int kunLaunchGraphBatch(KunBatchGraphHandle g, void *hip_stream);
 KUN_API void kunDestroyGraphBatch(KunBatchGraphHandle g);
 /* Blocks until the executor's stream is idle. */
 KUN_API int kunExecutorSynchronize(KunExecutorHandle exec);
+
+/* ---- session pools (no counterpart in CApi.h) ---------------------------
+ * Many stream sessions of one STREAM module resident in HBM side by side.
+ * A pool has `capacity_stocks` lanes; a session takes a 64-lane aligned
+ * range at admission (first fit) and returns it at release, so admission is
+ * by state footprint: kunStreamPoolBytesPerStock bytes of rows and state per
+ * lane. A cycle ticks a set of idle sessions once each (every session keeps
+ * its own tick count), in a fixed number of launches whatever the number of
+ * sessions; up to `max_inflight` cycles with disjoint sessions run at once.
+ * Per session, results equal those of a stream context of the same module
+ * fed the same rows. A pool is not thread-safe. */
+KUN_API size_t kunStreamPoolBytesPerStock(KunModuleHandle m);
+/* NULL on failure (kunGetLastError) */
+KUN_API KunStreamPoolHandle kunCreateStreamPool(KunExecutorHandle exec, KunModuleHandle m, size_t capacity_stocks,
+                                                int max_sessions, int max_inflight);
+KUN_API void kunDestroyStreamPool(KunStreamPoolHandle pool);
+KUN_API size_t kunStreamPoolFreeStocks(KunStreamPoolHandle pool);
+/* 0: admitted, *session set; 2: no room now (retry after a release); 1: error */
+KUN_API int kunStreamPoolAdmit(KunStreamPoolHandle pool, size_t num_stocks, int *session);
+/* waits for the session's cycle in flight, if any */
+KUN_API int kunStreamPoolRelease(KunStreamPoolHandle pool, int session);
+/* One tick of each of `nsessions` sessions (none in a cycle in flight, none
+ * twice). input_rows[j * ninputs + q] is the host row of module input q (in
+ * buffer order) of sessions[j]; the rows are copied before the call
+ * returns. Enqueues the cycle and returns its id in *cycle. */
+KUN_API int kunStreamPoolSubmit(KunStreamPoolHandle pool, int nsessions, const int *sessions,
+                                const void *const *input_rows, unsigned long long *cycle);
+/* 1: cycle finished, 0: running (wait = 0 only), -1: error */
+KUN_API int kunStreamPoolDone(KunStreamPoolHandle pool, unsigned long long cycle, int wait);
+/* current row of a buffer of a session: host copy (waits for the session's
+ * cycle) or device address (valid until the session's next cycle) */
+KUN_API int kunStreamPoolRead(KunStreamPoolHandle pool, int session, size_t buffer_handle, void *host_row);
+KUN_API const void *kunStreamPoolRowDevice(KunStreamPoolHandle pool, int session, size_t buffer_handle);
+/* ticks submitted for the session so far; -1 on error */
+KUN_API long kunStreamPoolTicks(KunStreamPoolHandle pool, int session);
 
 /* Module introspection. dtype: 0 float, 1 double. layouts: 0 STs, 1 TS, 2 STREAM. */
 KUN_API int kunModuleDtype(KunModuleHandle m);

@@ -14,7 +14,7 @@ RUNTIME_LIB = os.path.join(LIB_DIR, "libkunhip.so")
 MODULE_DIR = os.path.join(LIB_DIR, "modules")
 ARCH = os.environ.get("KUN_OFFLOAD_ARCH", "gfx950")
 
-RUNTIME_SOURCES = ["runtime.hip", "stream.hip", "cross_section.hip", "capi.cpp"]
+RUNTIME_SOURCES = ["runtime.hip", "stream.hip", "stream_pool.hip", "cross_section.hip", "capi.cpp"]
 
 
 def hipcc() -> str:

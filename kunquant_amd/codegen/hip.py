@@ -451,13 +451,15 @@ class StageEmitter(_Lowering):
         return L
 
     def stream_kernels(self) -> str:
+        """Per-stock tick and init bodies of the stage (device functions of
+        (stage arguments, stock s, tick i); A.S is the stride of the state
+        arrays), and the single-context kernels wrapping them: one lane per
+        stock of [0, A.S)."""
         T = self.T
-        tick = [f'extern "C" __global__ __launch_bounds__({TS_BLOCK}) void {self.kname}_tick(kun::StreamArgs A) {{',
-                f"  const long s = (long)blockIdx.x * {TS_BLOCK} + threadIdx.x;",
-                "  if (s >= A.S) return;",
+        tick = [f"__device__ __forceinline__ void {self.kname}_body(const kun::StreamArgs &A, const long s, "
+                f"const long i) {{",
                 f"  {T} *__restrict__ stT = ({T} *)A.st;",
-                f"  int32_t *__restrict__ istp = (int32_t *)(stT + (size_t){self.n_t} * A.S);",
-                "  const long i = A.tick;"]
+                f"  int32_t *__restrict__ istp = (int32_t *)(stT + (size_t){self.n_t} * A.S);"]
         tick.extend("  " + x for x in self.buffer_ptrs())
         tick.extend(self.state_decls())
         tick.append("  {")
@@ -465,9 +467,12 @@ class StageEmitter(_Lowering):
         tick.append("  }")
         tick.extend(self.state_stores())
         tick.append("}")
-        init = [f'extern "C" __global__ __launch_bounds__({TS_BLOCK}) void {self.kname}_init(kun::StreamArgs A) {{',
-                f"  const long s = (long)blockIdx.x * {TS_BLOCK} + threadIdx.x;",
-                "  if (s >= A.S) return;",
+        tick += [f'extern "C" __global__ __launch_bounds__({TS_BLOCK}) void {self.kname}_tick(kun::StreamArgs A) {{',
+                 f"  const long s = (long)blockIdx.x * {TS_BLOCK} + threadIdx.x;",
+                 "  if (s >= A.S) return;",
+                 f"  {self.kname}_body(A, s, A.tick);",
+                 "}"]
+        init = [f"__device__ __forceinline__ void {self.kname}_init_body(const kun::StreamArgs &A, const long s) {{",
                 f"  {T} *__restrict__ stT = ({T} *)A.st;",
                 f"  int32_t *__restrict__ istp = (int32_t *)(stT + (size_t){self.n_t} * A.S);"]
         for k, (slots, base) in sorted(self.ring.items()):
@@ -494,6 +499,11 @@ class StageEmitter(_Lowering):
                             f"{self.imem(f'{is_} + q')} = 0; }}")
                 init.append(f"  {self.imem(is_ + w + 1)} = 0; {self.imem(is_ + w + 2)} = 0;")
         init.append("}")
+        init += [f'extern "C" __global__ __launch_bounds__({TS_BLOCK}) void {self.kname}_init(kun::StreamArgs A) {{',
+                 f"  const long s = (long)blockIdx.x * {TS_BLOCK} + threadIdx.x;",
+                 "  if (s >= A.S) return;",
+                 f"  {self.kname}_init_body(A, s);",
+                 "}"]
         return "\n".join(tick) + "\n\n" + "\n".join(init)
 
 
@@ -971,6 +981,53 @@ def stage_plan(module: ModuleIR, stage) -> StagePlan:
     return plan_stage(stage.func.ops, force_seq=os.environ.get("KUN_FORCE_SEQ", "") == "1")
 
 
+def stream_phases(module: ModuleIR):
+    """Launch plan of a STREAM module's session pool: [(kind, stages)] with
+    kind 0 -- one kernel in which each stock's lane runs the tick bodies of
+    the listed time-series stages in order -- and kind 1 -- the listed
+    cross-sectional stages, one workgroup per (stage, session). Phases
+    alternate: a time-series stage joins the phase of its time-series
+    producers (its lane computes them first) and follows its cross-sectional
+    producers; a cross-sectional stage follows all its producers."""
+    writer = {}
+    for st in module.stages:
+        for b in st.out_bufs:
+            writer[b.idx] = st
+    order, seen = [], set()
+
+    def visit(st):
+        if st.idx in seen:
+            return
+        seen.add(st.idx)
+        for b in st.in_bufs:
+            w = writer.get(b.idx)
+            if w is not None and w is not st:
+                visit(w)
+        order.append(st)
+
+    for st in module.stages:
+        visit(st)
+    phase = {}
+    for st in order:
+        ts = st.kind == STAGE_TS
+        p = 0 if ts else 1
+        for b in st.in_bufs:
+            w = writer.get(b.idx)
+            if w is None or w is st:
+                continue
+            need = phase[w.idx] + (0 if ts and w.kind == STAGE_TS else 1)
+            if need % 2 != (0 if ts else 1):
+                need += 1
+            p = max(p, need)
+        phase[st.idx] = p
+    phases = []
+    for p in range(max(phase.values()) + 1 if phase else 0):
+        sts = [st for st in order if phase[st.idx] == p]
+        if sts:
+            phases.append((p % 2, sts))
+    return phases
+
+
 def emit_module(module: ModuleIR, lds_budget: int = None) -> str:
     """Full HIP translation unit for one module: kernels + descriptor.
     `lds_budget` overrides the LDS bytes per workgroup given to history
@@ -1040,6 +1097,34 @@ def emit_module(module: ModuleIR, lds_budget: int = None) -> str:
                              f"{lk.kname}_kdeps, {len(lk.kdeps)}, {lk.kname}_mreads, {len(lk.mat_reads)}, "
                              f"{lk.kname}_mwrites, {len(lk.mat_writes)}}},")
             parts.append("};")
+    pool = "0, nullptr, nullptr"
+    if module.stream_mode:
+        phases = stream_phases(module)
+        stateful = [st for st, _, _, _, sbytes, _ in meta if st.kind == STAGE_TS and sbytes]
+        for p, (kind, sts) in enumerate(phases):
+            parts.append(f"static const int kun_{mid}_ph{p}_stages[] = {{{', '.join(str(st.idx) for st in sts)}}};")
+            if kind == 0:
+                parts.append(f'extern "C" __global__ __launch_bounds__({TS_BLOCK}) void kun_{mid}_ph{p}'
+                             f"(kun::PoolArgs P) {{")
+                parts.append("  const kun::PoolBlock e = P.blocks[blockIdx.x];")
+                parts.append("  const long s = (long)e.s0 + threadIdx.x;")
+                parts.append("  if (s >= e.s1) return;")
+                for st in sts:
+                    parts.append(f"  kun_{mid}_s{st.idx}_body(P.stages[{st.idx}], s, e.tick);")
+                parts.append("}")
+        parts.append(f'extern "C" __global__ __launch_bounds__({TS_BLOCK}) void kun_{mid}_pinit(kun::PoolArgs P) {{')
+        parts.append("  const kun::PoolBlock e = P.blocks[blockIdx.x];")
+        parts.append("  const long s = (long)e.s0 + threadIdx.x;")
+        parts.append("  if (s >= e.s1) return;")
+        for st in stateful:
+            parts.append(f"  kun_{mid}_s{st.idx}_init_body(P.stages[{st.idx}], s);")
+        parts.append("}")
+        parts.append(f"static const KunPhaseDesc kun_{mid}_phases[] = {{")
+        for p, (kind, sts) in enumerate(phases):
+            kern = f"(const void *)&kun_{mid}_ph{p}" if kind == 0 else "nullptr"
+            parts.append(f"  {{{kind}, {kern}, kun_{mid}_ph{p}_stages, {len(sts)}}},")
+        parts.append("};")
+        pool = f"{len(phases)}, kun_{mid}_phases, (const void *)&kun_{mid}_pinit"
     if module.stream_mode:
         parts.append(f"static const void *kun_{mid}_inits[] = {{"
                      + ", ".join(f"(const void *)&{ini}" if ini else "nullptr" for _, _, _, ini, _, _ in meta) + "};")
@@ -1058,5 +1143,5 @@ def emit_module(module: ModuleIR, lds_budget: int = None) -> str:
     parts.append(f"  KUN_MODULE_ABI_VERSION, \"{module.name}\", {0 if module.dtype == 'float' else 1},"
                  f" {lay[module.input_layout]}, {lay[module.output_layout]}, {module.blocking_len},"
                  f" {1 if module.aligned else 0}, {len(module.buffers)}, kun_{mid}_bufs, {len(module.stages)},"
-                 f" kun_{mid}_stages, kun_{mid}_program, {inits}}};")
+                 f" kun_{mid}_stages, kun_{mid}_program, {inits}, {pool}}};")
     return "\n".join(parts) + "\n"

@@ -376,4 +376,62 @@ KUN_API size_t kunStreamTicks(KunStreamContextHandle context) { return stream_ti
 
 KUN_API void kunDestoryStream(KunStreamContextHandle context) { stream_destroy((StreamCtx *)context); }
 
+// ---- session pools ------------------------------------------------------
+
+KUN_API size_t kunStreamPoolBytesPerStock(KunModuleHandle m) {
+    size_t b = 0;
+    guarded([&] { b = pool_bytes_per_stock(unwrap_mod(m)); });
+    return b;
+}
+
+KUN_API KunStreamPoolHandle kunCreateStreamPool(KunExecutorHandle exec, KunModuleHandle m, size_t capacity_stocks,
+                                                int max_sessions, int max_inflight) {
+    StreamPool *p = nullptr;
+    guarded([&] { p = pool_create(*unwrap_exec(exec), unwrap_mod(m), capacity_stocks, max_sessions, max_inflight); });
+    return p;
+}
+
+KUN_API void kunDestroyStreamPool(KunStreamPoolHandle pool) { pool_destroy((StreamPool *)pool); }
+
+KUN_API size_t kunStreamPoolFreeStocks(KunStreamPoolHandle pool) { return pool_free_stocks((StreamPool *)pool); }
+
+KUN_API int kunStreamPoolAdmit(KunStreamPoolHandle pool, size_t num_stocks, int *session) {
+    int sid = -1;
+    if (guarded([&] { sid = pool_admit((StreamPool *)pool, num_stocks); })) return 1;
+    if (sid < 0) return 2;
+    *session = sid;
+    return 0;
+}
+
+KUN_API int kunStreamPoolRelease(KunStreamPoolHandle pool, int session) {
+    return guarded([&] { pool_release((StreamPool *)pool, session); });
+}
+
+KUN_API int kunStreamPoolSubmit(KunStreamPoolHandle pool, int nsessions, const int *sessions,
+                                const void *const *input_rows, unsigned long long *cycle) {
+    return guarded([&] { *cycle = pool_submit((StreamPool *)pool, nsessions, sessions, input_rows); });
+}
+
+KUN_API int kunStreamPoolDone(KunStreamPoolHandle pool, unsigned long long cycle, int wait) {
+    int r = -1;
+    if (guarded([&] { r = pool_done((StreamPool *)pool, cycle, wait != 0); })) return -1;
+    return r;
+}
+
+KUN_API int kunStreamPoolRead(KunStreamPoolHandle pool, int session, size_t buffer_handle, void *host_row) {
+    return guarded([&] { pool_read((StreamPool *)pool, session, buffer_handle, host_row); });
+}
+
+KUN_API const void *kunStreamPoolRowDevice(KunStreamPoolHandle pool, int session, size_t buffer_handle) {
+    const void *r = nullptr;
+    guarded([&] { r = pool_row_device((StreamPool *)pool, session, buffer_handle); });
+    return r;
+}
+
+KUN_API long kunStreamPoolTicks(KunStreamPoolHandle pool, int session) {
+    long t = -1;
+    guarded([&] { t = pool_ticks((StreamPool *)pool, session); });
+    return t;
+}
+
 } // extern "C"

@@ -64,7 +64,8 @@ template <typename K, int BLOCK, int ITEMS> struct PairSort<K, BLOCK, ITEMS, tru
 // either sort serves, since ranks only need every element inside its tie
 // group.
 template <typename T, int BLOCK, int ITEMS, bool MERGE>
-__global__ __launch_bounds__(BLOCK) void rank_rows(kun_cs::View in, kun_cs::View out, long S, int blk) {
+__device__ __forceinline__ void rank_row(const kun_cs::View &in, const kun_cs::View &out, long S, int blk,
+                                         const long t) {
     using K = typename KeyOf<T>::K;
     constexpr int N = BLOCK * ITEMS;
     using Sort = PairSort<K, BLOCK, ITEMS, MERGE>;
@@ -78,7 +79,6 @@ __global__ __launch_bounds__(BLOCK) void rank_rows(kun_cs::View in, kun_cs::View
     __shared__ int nvalid;
     int *endof = (int *)sh.skey;
     T *res = (T *)sh.skey;
-    const long t = blockIdx.x;
     const T *src = (const T *)in.ptr;
     T *dst = (T *)out.ptr;
     const long rin = t + in.row0;
@@ -148,6 +148,20 @@ __global__ __launch_bounds__(BLOCK) void rank_rows(kun_cs::View in, kun_cs::View
     for (long s = threadIdx.x; s < S; s += BLOCK) dst[cs_at(out.layout, t + out.row0, s, S, out.rows, blk)] = res[s];
 }
 
+// one workgroup per row of the views
+template <typename T, int BLOCK, int ITEMS, bool MERGE>
+__global__ __launch_bounds__(BLOCK) void rank_rows(kun_cs::View in, kun_cs::View out, long S, int blk) {
+    rank_row<T, BLOCK, ITEMS, MERGE>(in, out, S, blk, blockIdx.x);
+}
+
+// one workgroup per segment (a contiguous run of stocks of one row)
+template <typename T, int BLOCK, int ITEMS, bool MERGE>
+__global__ __launch_bounds__(BLOCK) void rank_segs(const kun_cs::Seg *segs) {
+    const kun_cs::Seg g = segs[blockIdx.x];
+    const kun_cs::View in{const_cast<void *>(g.in), 2, 1, 0}, out{g.out, 2, 1, 0};
+    rank_row<T, BLOCK, ITEMS, MERGE>(in, out, g.n, 1, 0);
+}
+
 // SCALE_ROWS rows per workgroup; the sum of each row is one sequential chain
 // (lane r of the first wave owns row r), fed from LDS tiles that the whole
 // workgroup loads one tile ahead and stores already masked: |x|, or +0 for a
@@ -165,12 +179,11 @@ template <int LAY> __device__ __forceinline__ size_t lay_at(const kun_cs::View &
 }
 
 template <typename T, int LIN, int LOUT>
-__global__ __launch_bounds__(SCALE_BLOCK) void scale_rows(kun_cs::View in, kun_cs::View out, long S, long rows,
-                                                           int blk) {
+__device__ __forceinline__ void scale_block(const kun_cs::View &in, const kun_cs::View &out, long S, long rows,
+                                            int blk, const long r0) {
     constexpr int PER = SCALE_ROWS * SCALE_TILE / SCALE_BLOCK;
     __shared__ T tile[SCALE_ROWS * (SCALE_TILE + 1)];
     __shared__ T total[SCALE_ROWS];
-    const long r0 = (long)blockIdx.x * SCALE_ROWS;
     const int nr = (int)((rows - r0) < SCALE_ROWS ? (rows - r0) : SCALE_ROWS);
     const T *src = (const T *)in.ptr;
     T *dst = (T *)out.ptr;
@@ -216,6 +229,18 @@ __global__ __launch_bounds__(SCALE_BLOCK) void scale_rows(kun_cs::View in, kun_c
     }
 }
 
+template <typename T, int LIN, int LOUT>
+__global__ __launch_bounds__(SCALE_BLOCK) void scale_rows(kun_cs::View in, kun_cs::View out, long S, long rows,
+                                                           int blk) {
+    scale_block<T, LIN, LOUT>(in, out, S, rows, blk, (long)blockIdx.x * SCALE_ROWS);
+}
+
+template <typename T> __global__ __launch_bounds__(SCALE_BLOCK) void scale_segs(const kun_cs::Seg *segs) {
+    const kun_cs::Seg g = segs[blockIdx.x];
+    const kun_cs::View in{const_cast<void *>(g.in), 2, 1, 0}, out{g.out, 2, 1, 0};
+    scale_block<T, 2, 2>(in, out, g.n, 1, 1, 0);
+}
+
 template <typename T>
 hipError_t launch_scale_t(const kun_cs::View &in, const kun_cs::View &out, long S, long rows, int blk,
                           hipStream_t st) {
@@ -256,9 +281,46 @@ hipError_t launch_rank_t(const kun_cs::View &in, const kun_cs::View &out, long S
     return hipErrorInvalidValue;
 }
 
+template <typename T, int BLOCK, int ITEMS, bool MERGE>
+hipError_t launch_rank_segs_n(const kun_cs::Seg *segs, int nseg, hipStream_t st) {
+    hipLaunchKernelGGL((rank_segs<T, BLOCK, ITEMS, MERGE>), dim3((unsigned)nseg), dim3(BLOCK), 0, st, segs);
+    return hipGetLastError();
+}
+
+template <typename T> hipError_t launch_rank_segs_t(const kun_cs::Seg *segs, int nseg, long maxS, hipStream_t st) {
+    if (maxS <= 256) return launch_rank_segs_n<T, 256, 1, false>(segs, nseg, st);
+    if (maxS <= 512) return launch_rank_segs_n<T, 256, 2, false>(segs, nseg, st);
+    if (maxS <= 1024) return launch_rank_segs_n<T, 256, 4, false>(segs, nseg, st);
+    if (maxS <= 2048) return launch_rank_segs_n<T, 1024, 2, true>(segs, nseg, st);
+    if (maxS <= 4096) return launch_rank_segs_n<T, 1024, 4, true>(segs, nseg, st);
+    if (maxS <= 8192) return launch_rank_segs_n<T, 1024, 8, true>(segs, nseg, st);
+    if (maxS <= 16384) {
+        if constexpr (sizeof(T) == 8)
+            return launch_rank_segs_n<T, 512, 32, false>(segs, nseg, st);
+        else
+            return launch_rank_segs_n<T, 1024, 16, true>(segs, nseg, st);
+    }
+    return hipErrorInvalidValue;
+}
+
 } // namespace
 
 namespace kun_cs {
+
+hipError_t launch_rank_segs(int dtype, const Seg *segs, int nseg, long maxS, hipStream_t st) {
+    if (nseg <= 0) return hipSuccess;
+    return dtype ? launch_rank_segs_t<double>(segs, nseg, maxS, st) : launch_rank_segs_t<float>(segs, nseg, maxS, st);
+}
+
+hipError_t launch_scale_segs(int dtype, const Seg *segs, int nseg, hipStream_t st) {
+    if (nseg <= 0) return hipSuccess;
+    if (dtype)
+        hipLaunchKernelGGL((scale_segs<double>), dim3((unsigned)nseg), dim3(SCALE_BLOCK), 0, st, segs);
+    else
+        hipLaunchKernelGGL((scale_segs<float>), dim3((unsigned)nseg), dim3(SCALE_BLOCK), 0, st, segs);
+    return hipGetLastError();
+}
+
 
 long max_rank_stocks(int dtype) { return 16384; }
 

@@ -14,7 +14,18 @@ struct View {
     long row0;
 };
 
+// A contiguous run of n stocks of one row (layout 2), e.g. one session's
+// range of a pool row; `in` and `out` point at its first stock.
+struct Seg {
+    const void *in;
+    void *out;
+    long n;
+};
+
 long max_rank_stocks(int dtype);
+// one workgroup per segment; maxS bounds every segment's n
+hipError_t launch_rank_segs(int dtype, const Seg *segs, int nseg, long maxS, hipStream_t st);
+hipError_t launch_scale_segs(int dtype, const Seg *segs, int nseg, hipStream_t st);
 hipError_t launch_rank(int dtype, const View &in, const View &out, long S, long rows, int blk, hipStream_t st);
 hipError_t launch_scale(int dtype, const View &in, const View &out, long S, long rows, int blk, hipStream_t st);
 

@@ -374,4 +374,17 @@ struct StreamArgs {
     long S;
     long tick;
 };
+// Session pools: one entry per workgroup of a pool launch -- the lanes
+// [s0, s1) of one session's stock range (at most one wavefront), where the
+// session's rows start in the packed input rows of the tick (p0), and the
+// session's tick counter.
+struct PoolBlock {
+    int s0, s1;
+    long p0;
+    long tick;
+};
+struct PoolArgs {
+    const StreamArgs *stages; // per stage of the module (device memory)
+    const PoolBlock *blocks;
+};
 } // namespace kun

@@ -8,13 +8,26 @@
 #ifndef KUN_MODULE_ABI_H
 #define KUN_MODULE_ABI_H
 
-#define KUN_MODULE_ABI_VERSION 0x4B554E06u
+#define KUN_MODULE_ABI_VERSION 0x4B554E07u
 
 enum { KUN_BUF_INPUT = 0, KUN_BUF_OUTPUT = 1, KUN_BUF_TEMP = 2 };
 enum { KUN_STAGE_TS = 0, KUN_STAGE_RANK = 1, KUN_STAGE_SCALE = 2 };
 enum { KUN_LAYOUT_STS = 0, KUN_LAYOUT_TS = 1, KUN_LAYOUT_STREAM = 2 };
 enum { KUN_DTYPE_FLOAT = 0, KUN_DTYPE_DOUBLE = 1 };
 enum { KUN_KERNEL_PAR = 0, KUN_KERNEL_SEQ = 1 };
+
+enum { KUN_PHASE_TS = 0, KUN_PHASE_CS = 1 };
+
+/* One launch of a session pool's tick (STREAM modules): KUN_PHASE_TS, a
+ * kernel taking kun::PoolArgs in which every stock's lane runs the tick
+ * bodies of `stages` in order; KUN_PHASE_CS, the cross-sectional `stages`
+ * over every session's stock range (kernel is null). */
+typedef struct {
+    int kind;
+    const void *kernel;
+    const int *stages;
+    int nstages;
+} KunPhaseDesc;
 
 typedef struct {
     const char *name;
@@ -82,6 +95,11 @@ typedef struct {
     const KunStageDesc *stages;
     const char *program; /* KUNPROG text of the module */
     const void *const *stream_inits; /* per stage: state-initialising kernel (STREAM modules) */
+    /* STREAM modules, session pools: phases in launch order, and the kernel
+     * initialising every stage's state over the lanes of a block table */
+    int nphase;
+    const KunPhaseDesc *phases;
+    const void *pool_init;
 } KunModuleDesc;
 
 #endif

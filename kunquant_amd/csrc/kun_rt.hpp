@@ -174,4 +174,19 @@ const void *stream_current_device(StreamCtx *c, size_t handle);
 size_t stream_ticks(StreamCtx *c);
 std::string stream_serialize(StreamCtx *c);
 
+// ---- session pools (stream_pool.hip) ------------------------------------
+struct StreamPool;
+size_t pool_bytes_per_stock(const KunModuleDesc *m);
+StreamPool *pool_create(std::shared_ptr<Executor> ex, const KunModuleDesc *m, size_t capacity, int max_sessions,
+                        int max_inflight);
+void pool_destroy(StreamPool *p);
+size_t pool_free_stocks(StreamPool *p);
+int pool_admit(StreamPool *p, size_t S);
+void pool_release(StreamPool *p, int sid);
+unsigned long long pool_submit(StreamPool *p, int n, const int *sessions, const void *const *inputs);
+int pool_done(StreamPool *p, unsigned long long cycle, bool wait);
+const void *pool_row_device(StreamPool *p, int sid, size_t b);
+void pool_read(StreamPool *p, int sid, size_t b, void *host);
+long pool_ticks(StreamPool *p, int sid);
+
 } // namespace kunrt

@@ -1,0 +1,454 @@
+// Session pools: many stream sessions of one STREAM module resident in HBM
+// side by side, ticked together.
+//
+// A pool owns one stock space of `capacity` lanes. Every module buffer is one
+// row of `capacity` values and every time-series stage owns its carried
+// state (history rings, op state) as [slot][capacity] arrays, exactly the
+// layout of a single stream context with num_stocks = capacity. A session is
+// a range [s0, s0 + S) of that space (64-lane aligned), taken first-fit at
+// admission and returned at release, so the pool admits sessions by their
+// state footprint.
+//
+// A cycle ticks any set of idle sessions once, each with its own tick
+// counter: one host-to-device copy of the cycle's tables and packed input
+// rows, one scatter kernel, then per phase of the module's launch plan
+// (KunPhaseDesc) one time-series kernel over the blocks of every session in
+// the cycle, or one Rank and one Scale launch with a workgroup per (stage,
+// session). Results equal those of one stream context per session: the same
+// per-stock tick bodies and row kernels run on the same values.
+//
+// Cycles run on the pool's own streams; up to `max_inflight` cycles with
+// disjoint sessions are on the device at once (a session belongs to at most
+// one cycle in flight), so ticks that arrive while a cycle runs start
+// without waiting for it.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "kun_cs.hpp"
+#include "kun_device.hpp"
+#include "kun_module_abi.h"
+#include "kun_rt.hpp"
+
+namespace kunrt {
+
+namespace {
+
+constexpr int POOL_MAX_INPUTS = 16;
+
+struct ScatterArgs {
+    void *dst[POOL_MAX_INPUTS]; // pool rows of the module's inputs
+    const void *packed;         // [input][total] rows of the cycle's sessions
+    long total;
+    const kun::PoolBlock *blocks;
+    int nin;
+};
+
+template <typename T> __global__ __launch_bounds__(64) void pool_scatter(ScatterArgs a) {
+    const kun::PoolBlock e = a.blocks[blockIdx.x];
+    const long s = (long)e.s0 + threadIdx.x;
+    if (s >= e.s1) return;
+    const long k = e.p0 + threadIdx.x;
+    for (int q = 0; q < a.nin; q++) ((T *)a.dst[q])[s] = ((const T *)a.packed)[(size_t)q * a.total + k];
+}
+
+// NaN into [s0, s0 + n) of `nrow` rows of `stride` values
+template <typename T> __global__ void pool_fill_nan(T *rows, long stride, long s0, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) rows[(size_t)blockIdx.y * stride + s0 + i] = kun::qnan<T>();
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+} // namespace
+
+struct PoolSession {
+    bool live = false;
+    size_t s0 = 0, S = 0;
+    long tick = 0;
+    int slot = -1; // in-flight cycle slot, or -1
+};
+
+struct PoolSlot {
+    hipStream_t st = nullptr;
+    hipEvent_t done = nullptr;
+    char *host = nullptr; // pinned staging
+    char *dev = nullptr;
+    bool busy = false;
+    unsigned long long cycle = 0;
+    std::vector<int> sessions;
+};
+
+struct StreamPool {
+    std::shared_ptr<Executor> ex;
+    const KunModuleDesc *m = nullptr;
+    size_t cap = 0, esz = 4;
+    int max_sessions = 0;
+    char *rows = nullptr;                 // [nbuf][cap]
+    std::vector<void *> state;            // per stage [state_bytes][cap]
+    kun::StreamArgs *d_args = nullptr;    // per stage
+    std::vector<int> inputs;              // input buffer indices, module order
+    std::vector<int> ncs_rank, ncs_scale; // per phase: Rank / Scale stages
+    size_t stage_bytes = 0;               // per-stock bytes of rows + state
+    std::map<size_t, size_t> free_;       // start -> length, 64-lane units
+    std::vector<PoolSession> sess;
+    std::vector<PoolSlot> slots;
+    size_t staging = 0;
+    unsigned long long next_cycle = 1;
+    char *scratch = nullptr;              // admission block table
+    ~StreamPool();
+    void *row(int b) const { return rows + (size_t)b * cap * esz; }
+};
+
+StreamPool::~StreamPool() {
+    if (!ex) return;
+    hipSetDevice(ex->device);
+    for (auto &sl : slots) {
+        if (sl.st) hipStreamSynchronize(sl.st);
+        if (sl.st) hipStreamDestroy(sl.st);
+        if (sl.done) hipEventDestroy(sl.done);
+        if (sl.host) hipHostFree(sl.host);
+        if (sl.dev) hipFree(sl.dev);
+    }
+    hipStreamSynchronize(ex->stream);
+    if (rows) hipFree(rows);
+    for (void *p : state)
+        if (p) hipFree(p);
+    if (d_args) hipFree(d_args);
+    if (scratch) hipFree(scratch);
+}
+
+size_t pool_bytes_per_stock(const KunModuleDesc *m) {
+    size_t b = (size_t)m->nbuf * elem_size(m);
+    for (int si = 0; si < m->nstage; si++)
+        if (m->stages[si].kind == KUN_STAGE_TS) b += (size_t)m->stages[si].state_bytes;
+    return b;
+}
+
+static void retire(StreamPool *p, PoolSlot &sl) {
+    for (int sid : sl.sessions) p->sess[sid].slot = -1;
+    sl.sessions.clear();
+    sl.busy = false;
+}
+
+static void wait_slot(StreamPool *p, PoolSlot &sl) {
+    if (!sl.busy) return;
+    HIPCHECK(hipEventSynchronize(sl.done));
+    retire(p, sl);
+}
+
+StreamPool *pool_create(std::shared_ptr<Executor> ex, const KunModuleDesc *m, size_t capacity, int max_sessions,
+                        int max_inflight) {
+    if (!m || m->abi != KUN_MODULE_ABI_VERSION)
+        throw Error("The required version in the module does not match the runtime version");
+    if (m->out_layout != KUN_LAYOUT_STREAM || !m->phases) throw Error("Session pools need a STREAM module");
+    if (capacity == 0 || max_sessions < 1 || max_inflight < 1) throw Error("bad session pool parameters");
+    std::unique_ptr<StreamPool> p(new StreamPool());
+    p->ex = ex;
+    p->m = m;
+    p->cap = (capacity + 63) / 64 * 64;
+    if (p->cap > 0x7fffffffUL) throw Error("session pool capacity too large");
+    p->esz = elem_size(m);
+    p->max_sessions = max_sessions;
+    for (int b = 0; b < m->nbuf; b++)
+        if (m->bufs[b].kind == KUN_BUF_INPUT) p->inputs.push_back(b);
+    if ((int)p->inputs.size() > POOL_MAX_INPUTS) throw Error("session pools take at most 16 inputs");
+    size_t max_segs = 0;
+    for (int ph = 0; ph < m->nphase; ph++) {
+        const KunPhaseDesc &pd = m->phases[ph];
+        int nr = 0, ns = 0;
+        if (pd.kind == KUN_PHASE_CS)
+            for (int j = 0; j < pd.nstages; j++) (m->stages[pd.stages[j]].kind == KUN_STAGE_RANK ? nr : ns)++;
+        p->ncs_rank.push_back(nr);
+        p->ncs_scale.push_back(ns);
+        max_segs += (size_t)(nr + ns) * max_sessions;
+    }
+    p->stage_bytes = pool_bytes_per_stock(m);
+    std::lock_guard<std::mutex> g(ex->mu);
+    HIPCHECK(hipSetDevice(ex->device));
+    HIPCHECK(hipMalloc((void **)&p->rows, (size_t)m->nbuf * p->cap * p->esz));
+    p->state.assign(m->nstage, nullptr);
+    std::vector<kun::StreamArgs> args(m->nstage);
+    for (int si = 0; si < m->nstage; si++) {
+        const KunStageDesc &sd = m->stages[si];
+        kun::StreamArgs &a = args[si];
+        memset(&a, 0, sizeof a);
+        if (sd.kind != KUN_STAGE_TS) continue;
+        int n = 0;
+        for (int j = 0; j < sd.nin; j++) a.bufs[n++] = p->row(sd.in[j]);
+        for (int j = 0; j < sd.nout; j++) a.bufs[n++] = p->row(sd.out[j]);
+        if (sd.state_bytes) HIPCHECK(hipMalloc(&p->state[si], (size_t)sd.state_bytes * p->cap));
+        a.st = p->state[si];
+        a.S = (long)p->cap;
+    }
+    HIPCHECK(hipMalloc((void **)&p->d_args, sizeof(kun::StreamArgs) * m->nstage));
+    HIPCHECK(hipMemcpy(p->d_args, args.data(), sizeof(kun::StreamArgs) * m->nstage, hipMemcpyHostToDevice));
+    // staging: block table, segment tables, packed input rows
+    const size_t max_blocks = p->cap / 64;
+    p->staging = align256(max_blocks * sizeof(kun::PoolBlock)) + align256(max_segs * sizeof(kun_cs::Seg) + 1) +
+                 align256(p->inputs.size() * p->cap * p->esz + 1);
+    p->slots.resize(max_inflight);
+    for (auto &sl : p->slots) {
+        HIPCHECK(hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking));
+        HIPCHECK(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+        HIPCHECK(hipHostMalloc((void **)&sl.host, p->staging, hipHostMallocDefault));
+        HIPCHECK(hipMalloc((void **)&sl.dev, p->staging));
+    }
+    HIPCHECK(hipMalloc((void **)&p->scratch, align256(max_blocks * sizeof(kun::PoolBlock))));
+    p->free_[0] = p->cap / 64;
+    return p.release();
+}
+
+void pool_destroy(StreamPool *p) { delete p; }
+
+size_t pool_free_stocks(StreamPool *p) {
+    size_t n = 0;
+    for (auto &kv : p->free_) n += kv.second;
+    return n * 64;
+}
+
+// -1: no contiguous room (the caller retries after a release)
+int pool_admit(StreamPool *p, size_t S) {
+    if (S == 0) throw Error("a session needs at least one stock");
+    const KunModuleDesc *m = p->m;
+    if (m->aligned && S % (size_t)m->blocking) throw Error("num_stocks must be a multiple of blocking_len");
+    for (int si = 0; si < m->nstage; si++)
+        if (m->stages[si].kind == KUN_STAGE_RANK && (long)S > kun_cs::max_rank_stocks(m->dtype))
+            throw Error("Rank over more than " + std::to_string(kun_cs::max_rank_stocks(m->dtype)) +
+                        " stocks is not supported");
+    const size_t units = (S + 63) / 64;
+    auto it = p->free_.begin();
+    while (it != p->free_.end() && it->second < units) ++it;
+    if (it == p->free_.end()) return -1;
+    int sid = -1;
+    for (size_t i = 0; i < p->sess.size(); i++)
+        if (!p->sess[i].live) {
+            sid = (int)i;
+            break;
+        }
+    if (sid < 0) {
+        if ((int)p->sess.size() >= p->max_sessions) return -1;
+        sid = (int)p->sess.size();
+        p->sess.emplace_back();
+    }
+    const size_t start = it->first, len = it->second;
+    p->free_.erase(it);
+    if (len > units) p->free_[start + units] = len - units;
+    PoolSession &ss = p->sess[sid];
+    ss.live = true;
+    ss.s0 = start * 64;
+    ss.S = S;
+    ss.tick = 0;
+    ss.slot = -1;
+    // NaN rows and initial state over the session's lanes (the pool's
+    // streams touch other sessions only)
+    std::lock_guard<std::mutex> g(p->ex->mu);
+    HIPCHECK(hipSetDevice(p->ex->device));
+    hipStream_t st = p->ex->stream;
+    dim3 grid((unsigned)((S + 255) / 256), (unsigned)m->nbuf);
+    if (p->esz == 8)
+        hipLaunchKernelGGL(pool_fill_nan<double>, grid, dim3(256), 0, st, (double *)p->rows, (long)p->cap,
+                           (long)ss.s0, (long)S);
+    else
+        hipLaunchKernelGGL(pool_fill_nan<float>, grid, dim3(256), 0, st, (float *)p->rows, (long)p->cap,
+                           (long)ss.s0, (long)S);
+    HIPCHECK(hipGetLastError());
+    std::vector<kun::PoolBlock> blocks;
+    for (size_t b = 0; b < units; b++) {
+        kun::PoolBlock e;
+        e.s0 = (int)(ss.s0 + b * 64);
+        e.s1 = (int)std::min(ss.s0 + b * 64 + 64, ss.s0 + S);
+        e.p0 = 0;
+        e.tick = 0;
+        blocks.push_back(e);
+    }
+    HIPCHECK(hipMemcpyAsync(p->scratch, blocks.data(), blocks.size() * sizeof(kun::PoolBlock), hipMemcpyHostToDevice,
+                            st));
+    kun::PoolArgs pa{p->d_args, (const kun::PoolBlock *)p->scratch};
+    void *args[] = {&pa};
+    HIPCHECK(hipLaunchKernel(m->pool_init, dim3((unsigned)units), dim3(64), args, 0, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    return sid;
+}
+
+static PoolSession &live_session(StreamPool *p, int sid) {
+    if (sid < 0 || sid >= (int)p->sess.size() || !p->sess[sid].live) throw Error("no such session");
+    return p->sess[sid];
+}
+
+void pool_release(StreamPool *p, int sid) {
+    PoolSession &ss = live_session(p, sid);
+    if (ss.slot >= 0) wait_slot(p, p->slots[ss.slot]);
+    size_t start = ss.s0 / 64, len = (ss.S + 63) / 64;
+    ss.live = false;
+    auto nx = p->free_.lower_bound(start);
+    if (nx != p->free_.end() && nx->first == start + len) {
+        len += nx->second;
+        p->free_.erase(nx);
+    }
+    auto pv = p->free_.lower_bound(start);
+    if (pv != p->free_.begin()) {
+        --pv;
+        if (pv->first + pv->second == start) {
+            start = pv->first;
+            len += pv->second;
+            p->free_.erase(pv);
+        }
+    }
+    p->free_[start] = len;
+}
+
+// inputs[j * nin + q]: host row (S values) of input q of sessions[j]
+unsigned long long pool_submit(StreamPool *p, int n, const int *sessions, const void *const *inputs) {
+    const KunModuleDesc *m = p->m;
+    if (n < 1) throw Error("a cycle needs at least one session");
+    // a free slot: the oldest busy one is waited for if none is free
+    PoolSlot *sl = nullptr;
+    for (auto &x : p->slots) {
+        if (x.busy && hipEventQuery(x.done) == hipSuccess) retire(p, x);
+        if (!x.busy && !sl) sl = &x;
+    }
+    if (!sl) {
+        sl = &p->slots[0];
+        for (auto &x : p->slots)
+            if (x.cycle < sl->cycle) sl = &x;
+        wait_slot(p, *sl);
+    }
+    std::vector<char> seen(p->sess.size(), 0);
+    size_t total = 0, nblocks = 0;
+    long maxS = 0;
+    for (int j = 0; j < n; j++) {
+        PoolSession &ss = live_session(p, sessions[j]);
+        if (ss.slot >= 0) throw Error("session " + std::to_string(sessions[j]) + " is in a cycle in flight");
+        if (seen[sessions[j]]) throw Error("session " + std::to_string(sessions[j]) + " twice in one cycle");
+        seen[sessions[j]] = 1;
+        total += ss.S;
+        nblocks += (ss.S + 63) / 64;
+        maxS = std::max(maxS, (long)ss.S);
+    }
+    const size_t nin = p->inputs.size();
+    // staging layout
+    char *h = sl->host;
+    const size_t off_blocks = 0;
+    const size_t off_segs = align256(nblocks * sizeof(kun::PoolBlock));
+    size_t nseg = 0;
+    for (int ph = 0; ph < m->nphase; ph++) nseg += (size_t)(p->ncs_rank[ph] + p->ncs_scale[ph]) * n;
+    const size_t off_in = off_segs + align256(nseg * sizeof(kun_cs::Seg) + 1);
+    const size_t used = off_in + nin * total * p->esz;
+    if (used > p->staging) throw Error("session pool staging overflow");
+    kun::PoolBlock *blk = (kun::PoolBlock *)(h + off_blocks);
+    size_t off = 0, bi = 0;
+    for (int j = 0; j < n; j++) {
+        const PoolSession &ss = p->sess[sessions[j]];
+        for (size_t b = 0; b * 64 < ss.S; b++, bi++) {
+            blk[bi].s0 = (int)(ss.s0 + b * 64);
+            blk[bi].s1 = (int)std::min(ss.s0 + b * 64 + 64, ss.s0 + ss.S);
+            blk[bi].p0 = (long)(off + b * 64);
+            blk[bi].tick = ss.tick;
+        }
+        for (size_t q = 0; q < nin; q++) {
+            const void *src = inputs[(size_t)j * nin + q];
+            if (!src) throw Error("missing input row");
+            memcpy(h + off_in + (q * total + off) * p->esz, src, ss.S * p->esz);
+        }
+        off += ss.S;
+    }
+    kun_cs::Seg *seg = (kun_cs::Seg *)(h + off_segs);
+    size_t si = 0;
+    for (int ph = 0; ph < m->nphase; ph++) {
+        const KunPhaseDesc &pd = m->phases[ph];
+        if (pd.kind != KUN_PHASE_CS) continue;
+        for (int kind : {KUN_STAGE_RANK, KUN_STAGE_SCALE})
+            for (int k = 0; k < pd.nstages; k++) {
+                const KunStageDesc &sd = m->stages[pd.stages[k]];
+                if (sd.kind != kind) continue;
+                for (int j = 0; j < n; j++) {
+                    const PoolSession &ss = p->sess[sessions[j]];
+                    seg[si].in = (const char *)p->row(sd.in[0]) + ss.s0 * p->esz;
+                    seg[si].out = (char *)p->row(sd.out[0]) + ss.s0 * p->esz;
+                    seg[si].n = (long)ss.S;
+                    si++;
+                }
+            }
+    }
+    hipStream_t st = sl->st;
+    HIPCHECK(hipSetDevice(p->ex->device));
+    HIPCHECK(hipMemcpyAsync(sl->dev, h, used, hipMemcpyHostToDevice, st));
+    const kun::PoolBlock *dblk = (const kun::PoolBlock *)(sl->dev + off_blocks);
+    const kun_cs::Seg *dseg = (const kun_cs::Seg *)(sl->dev + off_segs);
+    ScatterArgs sa;
+    memset(&sa, 0, sizeof sa);
+    for (size_t q = 0; q < nin; q++) sa.dst[q] = p->row(p->inputs[q]);
+    sa.packed = sl->dev + off_in;
+    sa.total = (long)total;
+    sa.blocks = dblk;
+    sa.nin = (int)nin;
+    if (p->esz == 8)
+        hipLaunchKernelGGL(pool_scatter<double>, dim3((unsigned)nblocks), dim3(64), 0, st, sa);
+    else
+        hipLaunchKernelGGL(pool_scatter<float>, dim3((unsigned)nblocks), dim3(64), 0, st, sa);
+    HIPCHECK(hipGetLastError());
+    kun::PoolArgs pa{p->d_args, dblk};
+    void *args[] = {&pa};
+    size_t so = 0;
+    for (int ph = 0; ph < m->nphase; ph++) {
+        const KunPhaseDesc &pd = m->phases[ph];
+        if (pd.kind == KUN_PHASE_TS) {
+            HIPCHECK(hipLaunchKernel(pd.kernel, dim3((unsigned)nblocks), dim3(64), args, 0, st));
+            continue;
+        }
+        const int nr = p->ncs_rank[ph] * n, ns = p->ncs_scale[ph] * n;
+        hipError_t e = kun_cs::launch_rank_segs(m->dtype, dseg + so, nr, maxS, st);
+        if (e == hipSuccess) e = kun_cs::launch_scale_segs(m->dtype, dseg + so + nr, ns, st);
+        if (e != hipSuccess) throw Error(std::string("cross-sectional kernel: ") + hipGetErrorString(e));
+        so += (size_t)nr + ns;
+    }
+    HIPCHECK(hipEventRecord(sl->done, st));
+    sl->busy = true;
+    sl->cycle = p->next_cycle++;
+    for (int j = 0; j < n; j++) {
+        PoolSession &ss = p->sess[sessions[j]];
+        ss.tick++;
+        ss.slot = (int)(sl - p->slots.data());
+        sl->sessions.push_back(sessions[j]);
+    }
+    return sl->cycle;
+}
+
+// 1 when cycle `c` has finished (or is unknown, i.e. finished earlier)
+int pool_done(StreamPool *p, unsigned long long c, bool wait) {
+    for (auto &sl : p->slots)
+        if (sl.busy && sl.cycle == c) {
+            if (wait) {
+                wait_slot(p, sl);
+                return 1;
+            }
+            hipError_t e = hipEventQuery(sl.done);
+            if (e == hipErrorNotReady) return 0;
+            HIPCHECK(e);
+            retire(p, sl);
+            return 1;
+        }
+    return 1;
+}
+
+const void *pool_row_device(StreamPool *p, int sid, size_t b) {
+    const PoolSession &ss = live_session(p, sid);
+    if (b >= (size_t)p->m->nbuf) throw Error("bad buffer handle");
+    return (const char *)p->row((int)b) + ss.s0 * p->esz;
+}
+
+void pool_read(StreamPool *p, int sid, size_t b, void *host) {
+    PoolSession &ss = live_session(p, sid);
+    if (b >= (size_t)p->m->nbuf) throw Error("bad buffer handle");
+    if (ss.slot >= 0) wait_slot(p, p->slots[ss.slot]);
+    HIPCHECK(hipSetDevice(p->ex->device));
+    HIPCHECK(hipMemcpy(host, (const char *)p->row((int)b) + ss.s0 * p->esz, ss.S * p->esz, hipMemcpyDeviceToHost));
+}
+
+long pool_ticks(StreamPool *p, int sid) { return live_session(p, sid).tick; }
+
+} // namespace kunrt

@@ -72,6 +72,17 @@ def _load():
     sig("kunLaunchGraphBatch", i, vp, vp)
     sig("kunDestroyGraphBatch", None, vp)
     sig("kunModuleStageNumKernels", i, vp, i)
+    sig("kunStreamPoolBytesPerStock", sz, vp)
+    sig("kunCreateStreamPool", vp, vp, vp, sz, i, i)
+    sig("kunDestroyStreamPool", None, vp)
+    sig("kunStreamPoolFreeStocks", sz, vp)
+    sig("kunStreamPoolAdmit", i, vp, sz, ctypes.POINTER(i))
+    sig("kunStreamPoolRelease", i, vp, i)
+    sig("kunStreamPoolSubmit", i, vp, i, ctypes.POINTER(i), ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_ulonglong))
+    sig("kunStreamPoolDone", i, vp, ctypes.c_ulonglong, i)
+    sig("kunStreamPoolRead", i, vp, i, sz, vp)
+    sig("kunStreamPoolRowDevice", vp, vp, i, sz)
+    sig("kunStreamPoolTicks", ctypes.c_long, vp, i)
     ip = ctypes.POINTER(ctypes.c_int)
     sig("kunModuleKernelInfo", i, vp, i, i, ip, ip, ip, ip, ip)
     sig("kunModuleDeviceBytes", sz, vp, sz, sz, ip)
@@ -530,6 +541,101 @@ class StreamContext:
         if rc:
             raise RuntimeError("Failed to serialize states")
         return buf.raw[:size.value]
+
+
+class StreamPool:
+    """Many stream sessions of one STREAM module resident in HBM side by
+    side (no counterpart in the reference, which runs one StreamContext per
+    session). A session takes `num_stocks` lanes of the pool's
+    `capacity_stocks` at admit() and returns them at release(); submit()
+    ticks a set of idle sessions once each in a fixed number of launches and
+    returns a cycle id; per session, results equal those of a StreamContext
+    fed the same rows."""
+
+    def __init__(self, exec: Executor, mod: Module, capacity_stocks: int, max_sessions: int = 1024,
+                 max_inflight: int = 2):
+        L = _load()
+        self._exec = exec
+        self._mod = mod
+        self._h = L.kunCreateStreamPool(exec._h, mod._h, int(capacity_stocks), int(max_sessions), int(max_inflight))
+        if not self._h:
+            raise RuntimeError(_err())
+        self._inputs = mod.getInputNames()
+        self._handle = {n: j for j, (n, *_) in enumerate(mod.buffers)}
+        self._stocks: Dict[int, int] = {}
+
+    @staticmethod
+    def bytes_per_stock(mod: Module) -> int:
+        """HBM bytes one session lane holds (rows and carried state)."""
+        return _load().kunStreamPoolBytesPerStock(mod._h)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.kunDestroyStreamPool(self._h)
+            self._h = None
+
+    def free_stocks(self) -> int:
+        return _load().kunStreamPoolFreeStocks(self._h)
+
+    def admit(self, num_stocks: int):
+        """Session id, or None when no contiguous range of the pool is free."""
+        sid = ctypes.c_int(-1)
+        rc = _load().kunStreamPoolAdmit(self._h, int(num_stocks), ctypes.byref(sid))
+        if rc == 2:
+            return None
+        if rc:
+            raise RuntimeError(_err())
+        self._stocks[sid.value] = int(num_stocks)
+        return sid.value
+
+    def release(self, sid: int):
+        if _load().kunStreamPoolRelease(self._h, sid):
+            raise RuntimeError(_err())
+        self._stocks.pop(sid, None)
+
+    def submit(self, ticks) -> int:
+        """ticks: [(session id, {input name: row ndarray [S]})]; one tick of
+        each listed session. Returns the cycle id."""
+        n = len(ticks)
+        nin = len(self._inputs)
+        sids = (ctypes.c_int * n)()
+        rows = (ctypes.c_void_p * (n * nin))()
+        keep = []
+        dtype = self._mod.dtype
+        for j, (sid, data) in enumerate(ticks):
+            sids[j] = sid
+            S = self._stocks.get(sid)
+            for q, name in enumerate(self._inputs):
+                a = data[name]
+                if not isinstance(a, np.ndarray) or a.dtype != dtype or not a.flags.c_contiguous or a.shape != (S,):
+                    raise RuntimeError(f"Bad input row {name} for session {sid}")
+                keep.append(a)
+                rows[j * nin + q] = a.ctypes.data
+        cyc = ctypes.c_ulonglong(0)
+        if _load().kunStreamPoolSubmit(self._h, n, sids, rows, ctypes.byref(cyc)):
+            raise RuntimeError(_err())
+        return cyc.value
+
+    def done(self, cycle: int, wait: bool = False) -> bool:
+        r = _load().kunStreamPoolDone(self._h, cycle, 1 if wait else 0)
+        if r < 0:
+            raise RuntimeError(_err())
+        return r == 1
+
+    def read(self, sid: int, name: str) -> np.ndarray:
+        out = np.empty(self._stocks[sid], dtype=self._mod.dtype)
+        if _load().kunStreamPoolRead(self._h, sid, self._handle[name], out.ctypes.data):
+            raise RuntimeError(_err())
+        return out
+
+    def row_device(self, sid: int, name: str) -> int:
+        p = _load().kunStreamPoolRowDevice(self._h, sid, self._handle[name])
+        if not p:
+            raise RuntimeError(_err())
+        return p
+
+    def ticks(self, sid: int) -> int:
+        return _load().kunStreamPoolTicks(self._h, sid)
 
 
 class _Mem(ctypes.Structure):

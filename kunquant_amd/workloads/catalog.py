@@ -36,6 +36,9 @@ CATALOG: Dict[str, Tuple[Callable, dict]] = {
     "stream_f64": (stream_graph, dict(dtype="double", input_layout="STREAM", output_layout="STREAM")),
     "stream_f32": (stream_graph, dict(input_layout="STREAM", output_layout="STREAM",
                                       options={"opt_reduce": False})),
+    # serving workload: the 100-factor random graph in stream mode (one row per tick)
+    "serve_f32": (lambda: random_graph(100, 5000), dict(input_layout="STREAM", output_layout="STREAM",
+                                                        options={"opt_reduce": False})),
 }
 
 

@@ -35,9 +35,11 @@ def main():
     ap.add_argument("--days", type=int, default=1024)
     ap.add_argument("--cpu-rows", type=int, default=128, help="rows of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", choices=("factors", "jobs", "serve"), default="factors")
+    ap.add_argument("--workload", choices=("factors", "jobs", "serve", "precision"), default="factors")
     ap.add_argument("--sessions", type=int, default=256, help="--workload serve: concurrent sessions")
     ap.add_argument("--rate", type=float, default=20000.0, help="--workload serve: Poisson ticks/s, all sessions")
+    ap.add_argument("--rate-sweep", default="5000,10000,15000",
+                    help="--workload serve: further offered rates, reported beside --rate")
     ap.add_argument("--seconds", type=float, default=3.0, help="--workload serve: length of the arrival trace")
     ap.add_argument("--state-budget-gb", type=float, default=64.0,
                     help="--workload serve: HBM for session rows and state (admission limit)")
@@ -65,6 +67,9 @@ def main():
         return
     if args.workload == "serve":
         run_serve(args)
+        return
+    if args.workload == "precision":
+        run_precision(args)
         return
 
     import numpy as np
@@ -333,7 +338,7 @@ def run_serve(args):
     rows = {}
     for j in live:
         d = data.ohlcv(sizes[j], NR, seed=5000 + j, nan_frac=0.01)
-        rows[j] = [{k: np.ascontiguousarray(d[k][t]) for k in names} for t in range(NR)]
+        rows[j] = [pool.rows(server.sid[j], {k: np.ascontiguousarray(d[k][t]) for k in names}) for t in range(NR)]
     print(f"serve: {len(live)}/{len(sizes)} sessions admitted, {sum(sizes[j] for j in live)} lanes, "
           f"{bps} B/lane", file=sys.stderr, flush=True)
 
@@ -350,25 +355,37 @@ def run_serve(args):
     server.cycles = server.ticks_in_cycles = 0
 
     # open loop: Poisson arrivals over all sessions
-    n = int(args.rate * args.seconds)
-    arr = np.cumsum(rng.exponential(1.0 / args.rate, n))
-    who = rng.integers(0, len(live), n)
-    k = 0
-    start = time.perf_counter()
-    while k < n or server.inflight or server.ready:
-        now = time.perf_counter() - start
-        while k < n and arr[k] <= now:
-            j = live[who[k]]
-            server.push(j, rows[j][k % NR], start + arr[k])
-            k += 1
-        server.poll()
-        server.dispatch()
-    end = time.perf_counter()
-    lat = np.array(server.latencies) * 1e3
-    p50, p99 = float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
-    achieved = len(lat) / (end - start)
-    print(f"serve: p50 {p50:.3f} ms p99 {p99:.3f} ms, {achieved:.0f} ticks/s, "
-          f"{server.ticks_in_cycles / max(1, server.cycles):.1f} ticks/cycle", file=sys.stderr, flush=True)
+    def open_loop(rate):
+        server.latencies.clear()
+        server.cycles = server.ticks_in_cycles = 0
+        server.submit_s = 0.0
+        n = int(rate * args.seconds)
+        arr = np.cumsum(rng.exponential(1.0 / rate, n))
+        who = rng.integers(0, len(live), n)
+        k = 0
+        start = time.perf_counter()
+        while k < n or server.inflight or server.ready:
+            now = time.perf_counter() - start
+            while k < n and arr[k] <= now:
+                j = live[who[k]]
+                server.push(j, rows[j][k % NR], start + arr[k])
+                k += 1
+            server.poll()
+            server.dispatch()
+        end = time.perf_counter()
+        lat = np.array(server.latencies) * 1e3
+        r = {"offered_ticks_per_s": rate, "achieved_ticks_per_s": len(lat) / (end - start),
+             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+             "max_ms": float(lat.max()), "ticks_per_cycle": server.ticks_in_cycles / max(1, server.cycles),
+             "cycles": server.cycles, "host_submit_ms_per_cycle": server.submit_s / max(1, server.cycles) * 1e3}
+        print(f"serve: offered {rate:.0f}/s: p50 {r['p50_ms']:.3f} ms p99 {r['p99_ms']:.3f} ms, "
+              f"{r['achieved_ticks_per_s']:.0f} ticks/s, {r['ticks_per_cycle']:.1f} ticks/cycle, "
+              f"submit {r['host_submit_ms_per_cycle']:.3f} ms/cycle", file=sys.stderr, flush=True)
+        return r, lat
+
+    sweep = [open_loop(float(x))[0] for x in args.rate_sweep.split(",") if x]
+    main_r, lat = open_loop(args.rate)
+    p50, p99, achieved = main_r["p50_ms"], main_r["p99_ms"], main_r["achieved_ticks_per_s"]
 
     # plain per-session path: one stream context per session, one tick at a time
     plain = []
@@ -377,7 +394,7 @@ def run_serve(args):
         hs = {k_: st.queryBufferHandle(k_) for k_ in names}
         for t in range(40):
             t1 = time.perf_counter()
-            for k_, v in rows[j][t % NR].items():
+            for k_, v in zip(names, rows[j][t % NR].arrays):
                 st.pushData(hs[k_], v)
             st.run()
             if t >= 10:
@@ -402,9 +419,8 @@ def run_serve(args):
                                f"session pool",
                    "sessions": args.sessions, "admitted": len(live), "lanes": int(sum(sizes[j] for j in live)),
                    "state_budget_GB": args.state_budget_gb, "bytes_per_lane": bps},
-        "serving": {"p50_ms": p50, "p99_ms": p99, "max_ms": float(lat.max()),
-                    "offered_ticks_per_s": args.rate, "achieved_ticks_per_s": achieved,
-                    "ticks_per_cycle": server.ticks_in_cycles / max(1, server.cycles), "cycles": server.cycles,
+        "serving": {**main_r,
+                    "rate_sweep": sweep,
                     "saturated_ticks_per_s": 60 * len(live) / sat_s, "saturated_ms_per_cycle": sat_s / sat_cycles * 1e3,
                     "latency": "arrival (scheduled time) to the tick's results complete in HBM",
                     "plain_per_session": {"p50_ms": float(np.percentile(plain, 50)),
@@ -413,6 +429,103 @@ def run_serve(args):
                                           "what": "StreamContext per session: push 6 rows, run, one tick at a "
                                                   "time (8 sessions x 30 ticks)"},
                     "speedup_ticks_per_s_vs_plain": (60 * len(live) / sat_s) / (1e3 / float(plain.mean()))},
+    }
+    print(json.dumps(result), flush=True)
+
+
+def factor_errors(got, ref, floor=1e-6):
+    """Per factor (torch tensors [T][S] on the device): relative error
+    |x - r| / max(|r|, floor * max|r|) over cells where both are finite --
+    its maximum, its median and the fraction of cells within 1e-5 -- and
+    the cells where exactly one of the two is NaN."""
+    import torch
+    res = {}
+    for k, r in ref.items():
+        x = got[k].to(torch.float64)
+        fin = torch.isfinite(r) & torch.isfinite(x)
+        mask_diff = int((torch.isnan(r) != torch.isnan(x)).sum().item())
+        if not bool(fin.any()):
+            res[k] = (0.0, 0.0, 1.0, mask_diff)
+            continue
+        ra = r.abs()
+        scale = float(ra[fin].max().item())
+        den = torch.clamp(ra, min=max(scale * floor, 1e-300))
+        err = ((x - r).abs() / den)[fin]
+        res[k] = (float(err.max().item()), float(err.median().item()), float((err <= 1e-5).double().mean().item()),
+                  mask_diff)
+    return res
+
+
+def run_precision(args):
+    import numpy as np
+    import torch
+    from kunquant_amd.runner import KunRunner as kr
+    from kunquant_amd.workloads import catalog, data
+
+    S, T = 4096, 2520
+    d64 = data.ohlcv(S, T, seed=3, dtype=np.float64, price_scale=(1e3, 1e5), volume_scale=(1e6, 1e9))
+    ex = kr.createSingleThreadExecutor()
+    outs, times = {}, {}
+    for name in ("prec_f64", "prec_f32", "prec_f32x"):
+        _, mod = catalog.load(name)
+        t_dt = torch.float64 if mod.dtype == np.float64 else torch.float32
+        dev = {k: torch.from_numpy(d64[k].astype(mod.dtype)).to("cuda:0") for k in mod.getInputNames()}
+        o = {k: torch.empty((T, S), dtype=t_dt, device="cuda:0") for k in mod.getOutputNames()}
+        ptrs = {k: v.data_ptr() for k, v in list(dev.items()) + list(o.items())}
+        kr.runGraphDevice(ex, mod, ptrs, S, T, 0, T)
+        ex.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            kr.runGraphDevice(ex, mod, ptrs, S, T, 0, T)
+        ex.synchronize()
+        times[name] = (time.perf_counter() - t0) / args.steps * 1e3
+        outs[name] = o
+        del dev
+        ex.release_temps()
+        print(f"precision: {name} {times[name]:.1f} ms", file=sys.stderr, flush=True)
+    ref = outs["prec_f64"]
+    e32 = factor_errors(outs["prec_f32"], ref)
+    e32x = factor_errors(outs["prec_f32x"], ref)
+    names = sorted(ref)
+    a = np.array([e32[k][0] for k in names])
+    b = np.array([e32x[k][0] for k in names])
+    cells = len(names) * S * T
+
+    def summary(e):
+        mx = np.array([e[k][0] for k in names])
+        med = np.array([e[k][1] for k in names])
+        frac = np.array([e[k][2] for k in names])
+        return {"max_err": {"median": float(np.median(mx)), "p90": float(np.percentile(mx, 90)),
+                            "max": float(mx.max()), "factors_le_1e-5": int((mx <= 1e-5).sum())},
+                "median_cell_err": {"median": float(np.median(med)), "max": float(med.max()),
+                                    "factors_le_1e-5": int((med <= 1e-5).sum())},
+                "cells_within_1e-5": float(np.mean(frac))}
+    result = {
+        "metric": "per-factor max relative error vs fp64 (fp32 path, median over factors)",
+        "value": float(np.median(a)),
+        "unit": "relative",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 1,
+        "ms_per_step": times["prec_f32"],
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded OHLCV, prices scaled to 1e3..1e5, volumes to 1e6..1e9)",
+        "config": {"workload": f"150-factor precision graph (quantile, regression slope/R2, 60-row var/corr), "
+                               f"{S} stocks x {T} days",
+                   "stocks": S, "days": T, "factors": len(names)},
+        "precision": {
+            "error": "per cell |x - r| / max(|r|, 1e-6 max|r| of the factor), r = fp64 run of the same graph and "
+                     "algorithm (bit-exact with the fp64 oracle), over cells where both are finite; per factor its "
+                     "max and median, then summarised over the 150 factors",
+            "fp32": summary(e32), "fp32_fp64_finalize": summary(e32x),
+            "nan_mask_mismatch_cells": {"fp32": int(sum(e32[k][3] for k in names)),
+                                        "fp32_fp64_finalize": int(sum(e32x[k][3] for k in names))},
+            "cells_per_s": {k: cells / (v / 1e3) for k, v in times.items()},
+            "per_factor_max_err": {k: [e32[k][0], e32x[k][0]] for k in names},
+        },
     }
     print(json.dumps(result), flush=True)
 

@@ -159,6 +159,31 @@ KUN_API void kunDestroyGraphBatch(KunBatchGraphHandle g);
 /* Blocks until the executor's stream is idle. */
 KUN_API int kunExecutorSynchronize(KunExecutorHandle exec);
 
+/* ---- corrWith / aggregrate (RunGraph.hpp:15-19, 55-75; the reference
+ * exposes them to Python only, PyBinding.cpp:396-508) --------------------
+ * kunCorrWith: out[i][t - cur_time] = correlation over the stocks of row t of
+ * inputs[i] (or of its rank, rank_inputs != 0) with row t of corr_with,
+ * float; layout 0 STs ([S/8][T][8]), 1 TS ([T][S]). Host buffers. */
+KUN_API int kunCorrWith(KunExecutorHandle exec, int layout, int rank_inputs, int num_inputs,
+                        const float *const *inputs, const float *corr_with, float *const *outs, size_t num_stocks,
+                        size_t total_time, size_t cur_time, size_t length);
+/* device addresses, enqueued on hip_stream (NULL: the executor's stream) */
+KUN_API int kunCorrWithDevice(KunExecutorHandle exec, int layout, int rank_inputs, int num_inputs,
+                              const float *const *inputs, const float *corr_with, float *const *outs,
+                              size_t num_stocks, size_t total_time, size_t cur_time, size_t length, void *hip_stream);
+/* kunAggregrate: per aggregation i, rows cur_time .. cur_time + length - 1 of
+ * inputs[i] ([T][S], TS) are split into groups where labels[i] ([T])
+ * changes; outs[i * 7 + k] (k: sum, min, max, first, last, count, mean; NULL
+ * = not wanted) receive [groups][S]. dtype 0 float, 1 double. Host buffers. */
+KUN_API int kunAggregrate(KunExecutorHandle exec, int num_aggregrations, const void *const *inputs,
+                          const void *const *labels, int dtype, void *const *outs, size_t num_stocks,
+                          size_t total_time, size_t cur_time, size_t length);
+/* device addresses; every output holds out_rows rows (more groups: error) */
+KUN_API int kunAggregrateDevice(KunExecutorHandle exec, int num_aggregrations, const void *const *inputs,
+                                const void *const *labels, int dtype, void *const *outs, size_t out_rows,
+                                size_t num_stocks, size_t total_time, size_t cur_time, size_t length,
+                                void *hip_stream);
+
 /* ---- session pools (no counterpart in CApi.h) ---------------------------
  * Many stream sessions of one STREAM module resident in HBM side by side.
  * A pool has `capacity_stocks` lanes; a session takes a 64-lane aligned

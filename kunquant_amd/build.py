@@ -14,7 +14,8 @@ RUNTIME_LIB = os.path.join(LIB_DIR, "libkunhip.so")
 MODULE_DIR = os.path.join(LIB_DIR, "modules")
 ARCH = os.environ.get("KUN_OFFLOAD_ARCH", "gfx950")
 
-RUNTIME_SOURCES = ["runtime.hip", "stream.hip", "stream_pool.hip", "cross_section.hip", "capi.cpp"]
+RUNTIME_SOURCES = ["runtime.hip", "stream.hip", "stream_pool.hip", "cross_section.hip", "row_reduce.hip",
+                   "capi.cpp"]
 
 
 def hipcc() -> str:
@@ -64,6 +65,24 @@ def build_runtime(force: bool = True) -> str:
     _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-ldl"])
     os.replace(tmp, RUNTIME_LIB)
     return RUNTIME_LIB
+
+
+def compile_unit(src: str, workdir: str, i: int, extra_flags: Sequence[str] = ()) -> str:
+    """Write generated translation unit `i` of a module and compile it."""
+    os.makedirs(workdir, exist_ok=True)
+    path = os.path.join(workdir, f"part{i}.hip")
+    with open(path, "w") as f:
+        f.write(src)
+    obj = path + ".o"
+    _run([hipcc()] + hip_flags(extra_flags) + ["-c", path, "-o", obj])
+    return obj
+
+
+def link_module(objs: List[str], out_path: str) -> str:
+    tmp = out_path + ".tmp"
+    _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + list(objs))
+    os.replace(tmp, out_path)
+    return out_path
 
 
 def compile_module_sources(sources: List[str], out_path: str, workdir: str, jobs: int = 8,

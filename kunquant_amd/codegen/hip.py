@@ -26,7 +26,7 @@ from typing import Dict, List, Optional, Tuple
 
 from ..Driver import BUF_INPUT, BUF_TEMP, ModuleIR, STAGE_TS
 from ..Op import ConstantOp, ForeachBackWindow, Input, OpBase, Output, ReductionOp
-from .plan import PAR, Component, StagePlan, plan_stage
+from .plan import PAR, SEQ, Component, StagePlan, plan_stage
 from .program import literal, program_text
 
 TS_BLOCK = 64
@@ -40,6 +40,7 @@ PREFETCH_REGS = int(os.environ.get("KUN_PREFETCH_REGS", 64))
 SEQ_PREFETCH_MAX = int(os.environ.get("KUN_SEQ_PREFETCH_MAX", 16))
 # unroll factor of window loops
 LOOP_UNROLL = int(os.environ.get("KUN_LOOP_UNROLL", 4))
+SORTED_REGS = 256  # register budget (values) for the sorted windows of one sequential component
 
 _BIN = {"Add": "+", "Sub": "-", "Mul": "*", "Div": "/"}
 _CMP = {"GreaterThan": ">", "GreaterEqual": ">=", "LessThan": "<", "LessEqual": "<=", "Equals": "=="}
@@ -101,6 +102,8 @@ class _Lowering:
     storage hooks: win(), input_expr(), output_lines(), after_value(),
     lane_expr() and tmem()."""
 
+    loop_unroll = LOOP_UNROLL  # window loops: #pragma unroll factor (0: full)
+
     time_idx = "i"
 
     def _setup(self, module: ModuleIR, stage):
@@ -108,6 +111,11 @@ class _Lowering:
         self.st = stage
         self.T = "float" if module.dtype == "float" else "double"
         self.esz = 4 if self.T == "float" else 8
+        # option fp64_finalize (fp32 modules): window-loop sums and linear
+        # regression sums accumulate in double, rounded to float once at the
+        # end -- the steps where sum-of-squares cancellation bites
+        self.fin = self.T == "float" and bool(module.options.get("fp64_finalize", False))
+        self.AT = "double" if self.fin else self.T
         self.idx: Dict[OpBase, int] = {op: i for i, op in enumerate(stage.func.ops)}
         self.slot: Dict[int, int] = {}
         for b in list(stage.in_bufs) + list(stage.out_bufs):
@@ -203,17 +211,18 @@ class _Lowering:
             L.append(f"const {T} {out} = acc{a};")
         elif name == "WindowedLinearRegression":
             old = self.win(op.inputs[0], op.attrs["window"])
-            L.append(f"st{k}.step({old}, {ins[0]}, ({T}){op.attrs['window']});")
+            A = self.AT
+            L.append(f"st{k}.step(({A})({old}), ({A})({ins[0]}), ({A}){op.attrs['window']});")
             L.append(f"const {T} {out} = 0;")
         elif name.startswith("WindowedLinearRegression") and name.endswith("Impl"):
             src = self.idx[op.inputs[0]]
             w = op.inputs[0].attrs["window"]
             if name == "WindowedLinearRegressionRSqaureImpl":
-                L.append(f"const {T} {out} = st{src}.rsquare({w});")
+                L.append(f"const {T} {out} = ({T})st{src}.rsquare({w});")
             elif name == "WindowedLinearRegressionSlopeImpl":
-                L.append(f"const {T} {out} = st{src}.slope({w});")
+                L.append(f"const {T} {out} = ({T})st{src}.slope({w});")
             else:
-                L.append(f"const {T} {out} = st{src}.resi({w}, {ins[1]});")
+                L.append(f"const {T} {out} = ({T})st{src}.resi({w}, ({self.AT})({ins[1]}));")
         elif name == "SkipListState":
             L.append(f"st{k}.step({ins[0]}, {ins[1]}, (int32_t)({time_idx}), {self.lane_expr(k)});")
             L.append(f"const {T} {out} = 0;")
@@ -249,7 +258,7 @@ class _Lowering:
             name = r.__class__.__name__
             L.append(f"{T} v{k};")
             if name == "ReduceAdd":
-                L.append(f"{T} r{k}_v = 0, r{k}_c = 0;")
+                L.append(f"{self.AT} r{k}_v = 0, r{k}_c = 0;")
             elif name == "ReduceMul":
                 L.append(f"{T} r{k}_v = 1;")
             elif name == "ReduceMin":
@@ -269,7 +278,7 @@ class _Lowering:
             else:
                 raise RuntimeError(f"HIP emitter: unsupported reduction {name}")
         L.append("{")
-        L.append(f"#pragma unroll {LOOP_UNROLL}")
+        L.append(f"#pragma unroll {self.loop_unroll}" if self.loop_unroll else "#pragma unroll")
         L.append(f"for (int iter = {w - 1}; iter >= 0; --iter) {{")
         for op in body:
             L.extend("  " + x for x in self.op_lines(op))
@@ -278,7 +287,9 @@ class _Lowering:
             x = self.v(r.inputs[0])
             name = r.__class__.__name__
             if name == "ReduceAdd":  # kahanAdd, Ops.hpp:289-298
-                L.append(f"  {{ const {T} y = {x} - r{k}_c; const {T} t = r{k}_v + y; "
+                A = self.AT
+                xv = f"({A}){x}" if self.fin else x
+                L.append(f"  {{ const {A} y = {xv} - r{k}_c; const {A} t = r{k}_v + y; "
                          f"r{k}_c = (t - r{k}_v) - y; r{k}_v = t; }}")
             elif name == "ReduceMul":
                 L.append(f"  r{k}_v = r{k}_v * {x};")
@@ -302,7 +313,9 @@ class _Lowering:
         for r in reds:
             k = self.idx[r]
             name = r.__class__.__name__
-            if name in ("ReduceAdd", "ReduceMul", "ReduceMin", "ReduceMax", "ReduceDecayLinear"):
+            if name == "ReduceAdd" and self.fin:
+                L.append(f"v{k} = ({T})r{k}_v;")
+            elif name in ("ReduceAdd", "ReduceMul", "ReduceMin", "ReduceMax", "ReduceDecayLinear"):
                 L.append(f"v{k} = r{k}_v;")
             elif name in ("ReduceArgMax", "ReduceArgMin"):
                 L.append(f"v{k} = r{k}_i;")
@@ -338,10 +351,15 @@ class StageEmitter(_Lowering):
     rings and carried state are [slot][S] arrays in HBM."""
 
     time_idx = "0"
+    # window loops read their rings from HBM: 16 loads in flight per group
+    # (a full unroll costs more in registers than it saves in latency)
+    loop_unroll = 16
 
     def __init__(self, module: ModuleIR, stage, kname: str):
         assert module.stream_mode
         self._setup(module, stage)
+        if self.fin:
+            raise RuntimeError("option fp64_finalize is not supported in stream mode")
         self.kname = kname
         self.ops: List[OpBase] = stage.func.ops
         self.ring: Dict[int, Tuple[int, int]] = {}      # op -> (ring slots, first T slot)
@@ -575,12 +593,24 @@ class CompEmitter(_Lowering):
             self.ring[self.idx[op]] = (slots, n_t)
             n_t += slots
         n_i = 0
+        # sorted windows of a sequential component live in registers while
+        # they fit SORTED_REGS values (with row indices where an argmin reads
+        # them), else in HBM scratch columns
+        self.sorted_regs: Dict[int, bool] = {}
+        regs = 0
         for op in self.ops:
             if op.__class__.__name__ == "SkipListState":
                 w = op.attrs["window"]
-                self.tslot[self.idx[op]] = n_t
+                k = self.idx[op]
+                idx = any(o.__class__.__name__ == "SkipListArgMin" and o.inputs[0] is op for o in self.ops)
+                need = (w + 1) * (2 if idx else 1)
+                if kind == SEQ and regs + need <= SORTED_REGS:
+                    regs += need
+                    self.sorted_regs[k] = idx
+                    continue
+                self.tslot[k] = n_t
                 n_t += w + 1
-                self.islot[self.idx[op]] = n_i
+                self.islot[k] = n_i
                 n_i += w + 1
         self.n_t, self.n_i = n_t, n_i
         self.halo = self._halo() if kind == PAR else 0
@@ -677,7 +707,8 @@ class CompEmitter(_Lowering):
         return f"istp[(size_t)({slot}) * cst + col]"
 
     def lane_expr(self, k: int = None) -> str:
-        return "col"
+        # register-resident sorted windows take no column
+        return "0" if k in self.sorted_regs else "col"
 
     def src_load(self, op, row: str) -> str:
         """HBM load of row source `op` at run row `row` (0 <= row < L)."""
@@ -738,7 +769,7 @@ class CompEmitter(_Lowering):
             if name == "FastWindowedSum":
                 L.append(f"kun::FastSum<{T}> st{k}; st{k}.init({op.attrs['window']});")
             elif name == "WindowedLinearRegression":
-                L.append(f"kun::LinReg<{T}> st{k}; st{k}.init({op.attrs['window']});")
+                L.append(f"kun::LinReg<{self.AT}> st{k}; st{k}.init({op.attrs['window']});")
             elif name == "ExpMovingAvg":
                 L.append(f"kun::Ema<{T}> st{k};")
                 if len(op.inputs) > 1:
@@ -750,6 +781,9 @@ class CompEmitter(_Lowering):
                 L.append(f"{T} acc{k} = 0;")
             elif name == "SkipListState":
                 w = op.attrs["window"]
+                if k in self.sorted_regs:
+                    L.append(f"kun::SortedWinR<{T}, {w}, {'true' if self.sorted_regs[k] else 'false'}> st{k};")
+                    continue
                 ts, is_ = self.tslot[k], self.islot[k]
                 L.append(f"kun::SortedWin<{T}, {w}> st{k}; st{k}.init(stT + (size_t)({ts - self.lds_t}) * cst, "
                          f"istp + (size_t){is_} * cst, cst);")
@@ -1032,8 +1066,20 @@ def emit_module(module: ModuleIR, lds_budget: int = None) -> str:
     """Full HIP translation unit for one module: kernels + descriptor.
     `lds_budget` overrides the LDS bytes per workgroup given to history
     rings before they move to HBM."""
+    return emit_module_parts(module, 1, lds_budget)[0]
+
+
+def emit_module_parts(module: ModuleIR, nparts: int, lds_budget: int = None) -> List[str]:
+    """The module as up to `nparts` translation units compiled in parallel:
+    batch stage kernels spread over the first units by source size, the
+    last unit holding the descriptor (with declarations of the kernels'
+    host stubs). Stream modules stay one unit (their phase kernels inline
+    the stage bodies)."""
     mid = ident(module.name)
-    parts = ['#include "kun_device.hpp"', '#include "kun_module_abi.h"', ""]
+    header = ['#include "kun_device.hpp"', '#include "kun_module_abi.h"', ""]
+    parts = list(header)
+    split = nparts > 1 and not module.stream_mode
+    kernel_srcs: List[Tuple[str, str]] = []  # (kernel name, source) of batch kernels
     meta = []
     for st in module.stages:
         kname = f"kun_{mid}_s{st.idx}"
@@ -1064,8 +1110,11 @@ def emit_module(module: ModuleIR, lds_budget: int = None) -> str:
                     groups.extend(cl[g:g + MAX_COMPS] for g in range(0, len(cl), MAX_COMPS))
                 for group in groups:
                     lk = LevelKernel(module, st, plan, level, f"{kname}_k{len(kerns)}", lds_budget, group)
-                    parts.append(lk.source())
-                    parts.append("")
+                    if split:
+                        kernel_srcs.append((lk.kname, lk.source()))
+                    else:
+                        parts.append(lk.source())
+                        parts.append("")
                     lk.kdeps = sorted({j for j, prev in enumerate(kerns) if prev.produced & lk.consumed})
                     kerns.append(lk)
             meta.append((st, kerns, None, None, 0, len(plan.mat)))
@@ -1144,4 +1193,17 @@ def emit_module(module: ModuleIR, lds_budget: int = None) -> str:
                  f" {lay[module.input_layout]}, {lay[module.output_layout]}, {module.blocking_len},"
                  f" {1 if module.aligned else 0}, {len(module.buffers)}, kun_{mid}_bufs, {len(module.stages)},"
                  f" kun_{mid}_stages, kun_{mid}_program, {inits}, {pool}}};")
-    return "\n".join(parts) + "\n"
+    if not split:
+        return ["\n".join(parts) + "\n"]
+    n = min(nparts - 1, len(kernel_srcs)) or 1
+    units: List[List[str]] = [list(header) for _ in range(n)]
+    sizes = [0] * n
+    for name, src in sorted(kernel_srcs, key=lambda ks: -len(ks[1])):
+        j = sizes.index(min(sizes))
+        units[j] += [src, ""]
+        sizes[j] += len(src)
+    # host-only unit: the descriptor points at the kernels' host stubs, which
+    # the other units define; its device pass has nothing to compile
+    decls = [f'extern "C" __global__ void {name}(kun::StageArgs A);' for name, _ in kernel_srcs]
+    desc = header + ["#if !defined(__HIP_DEVICE_COMPILE__)"] + decls + [""] + parts[len(header):] + ["#endif"]
+    return ["\n".join(u) + "\n" for u in units] + ["\n".join(desc) + "\n"]

@@ -376,6 +376,47 @@ KUN_API size_t kunStreamTicks(KunStreamContextHandle context) { return stream_ti
 
 KUN_API void kunDestoryStream(KunStreamContextHandle context) { stream_destroy((StreamCtx *)context); }
 
+// ---- corrWith / aggregrate ----------------------------------------------
+
+KUN_API int kunCorrWith(KunExecutorHandle exec, int layout, int rank_inputs, int num_inputs,
+                        const float *const *inputs, const float *corr_with, float *const *outs, size_t num_stocks,
+                        size_t total_time, size_t cur_time, size_t length) {
+    return guarded([&] {
+        if (layout != 0 && layout != 1) throw Error("Unknown layout");
+        kunrt::corr_with(**unwrap_exec(exec), false, layout == 0, rank_inputs != 0, num_inputs, inputs, corr_with, outs,
+                  num_stocks, total_time, cur_time, length, 8, nullptr);
+    });
+}
+
+KUN_API int kunCorrWithDevice(KunExecutorHandle exec, int layout, int rank_inputs, int num_inputs,
+                              const float *const *inputs, const float *corr_with, float *const *outs,
+                              size_t num_stocks, size_t total_time, size_t cur_time, size_t length, void *hip_stream) {
+    return guarded([&] {
+        if (layout != 0 && layout != 1) throw Error("Unknown layout");
+        kunrt::corr_with(**unwrap_exec(exec), true, layout == 0, rank_inputs != 0, num_inputs, inputs, corr_with, outs,
+                  num_stocks, total_time, cur_time, length, 8, (hipStream_t)hip_stream);
+    });
+}
+
+KUN_API int kunAggregrate(KunExecutorHandle exec, int num_aggregrations, const void *const *inputs,
+                          const void *const *labels, int dtype, void *const *outs, size_t num_stocks,
+                          size_t total_time, size_t cur_time, size_t length) {
+    return guarded([&] {
+        aggregate(**unwrap_exec(exec), false, num_aggregrations, inputs, labels, dtype, outs, num_stocks, total_time,
+                  cur_time, length, 0, nullptr);
+    });
+}
+
+KUN_API int kunAggregrateDevice(KunExecutorHandle exec, int num_aggregrations, const void *const *inputs,
+                                const void *const *labels, int dtype, void *const *outs, size_t out_rows,
+                                size_t num_stocks, size_t total_time, size_t cur_time, size_t length,
+                                void *hip_stream) {
+    return guarded([&] {
+        aggregate(**unwrap_exec(exec), true, num_aggregrations, inputs, labels, dtype, outs, num_stocks, total_time,
+                  cur_time, length, out_rows, (hipStream_t)hip_stream);
+    });
+}
+
 // ---- session pools ------------------------------------------------------
 
 KUN_API size_t kunStreamPoolBytesPerStock(KunModuleHandle m) {

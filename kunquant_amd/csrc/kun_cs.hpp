@@ -27,6 +27,12 @@ long max_rank_stocks(int dtype);
 hipError_t launch_rank_segs(int dtype, const Seg *segs, int nseg, long maxS, hipStream_t st);
 hipError_t launch_scale_segs(int dtype, const Seg *segs, int nseg, hipStream_t st);
 hipError_t launch_rank(int dtype, const View &in, const View &out, long S, long rows, int blk, hipStream_t st);
+// row_reduce.hip: per-row correlation (rows start .. start + L - 1 of y; x
+// rows from xrow0; sts: STs layout) and per-stock label-group reductions
+hipError_t launch_corr(const float *x, int xsts, long xT, long xrow0, const float *y, int ysts, long yT, float *out,
+                       long S, long start, long L, int blk, hipStream_t st);
+hipError_t launch_aggregate(int dtype, const void *in, const void *labels, void *const *outs, long S, long start,
+                            long L, hipStream_t st);
 hipError_t launch_scale(int dtype, const View &in, const View &out, long S, long rows, int blk, hipStream_t st);
 
 } // namespace kun_cs

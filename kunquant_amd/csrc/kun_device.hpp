@@ -321,6 +321,88 @@ template <typename T, int W> struct SortedWin {
     }
 };
 
+// The same sorted window held in registers by a lane that walks every row
+// of the run (sequential batch kernels): every access uses a compile-time
+// slot, so insertion and removal are W + 1 selects instead of a chain of
+// dependent memory round trips. IDX keeps the row indices (SkipListArgMin).
+// Ordering is that of SortedWin: a new value goes after equal ones, a
+// removal takes the first equal one.
+template <typename T, int W, bool IDX> struct SortedWinR {
+    T v[W + 1];
+    int32_t ix[IDX ? W + 1 : 1];
+    int n = 0, last = 0;
+    __device__ __forceinline__ SortedWinR() {
+#pragma unroll
+        for (int k = 0; k <= W; k++) v[k] = T(0);
+#pragma unroll
+        for (int k = 0; k < (IDX ? W + 1 : 1); k++) ix[k] = 0;
+    }
+    __device__ __forceinline__ void step(T old, T cur, int32_t index, long) {
+        if (old == old) {
+            int p = 0;
+#pragma unroll
+            for (int k = 0; k <= W; k++) p += (k < n && v[k] < old) ? 1 : 0;
+            bool found = false;
+#pragma unroll
+            for (int k = 0; k <= W; k++) found = found || (k == p && k < n && v[k] == old);
+            if (found) {
+#pragma unroll
+                for (int k = 0; k < W; k++) {
+                    v[k] = k >= p ? v[k + 1] : v[k];
+                    if (IDX) ix[k] = k >= p ? ix[k + 1] : ix[k];
+                }
+                n--;
+            }
+        }
+        if (cur == cur) {
+            int p = 0;
+#pragma unroll
+            for (int k = 0; k <= W; k++) p += (k < n && v[k] <= cur) ? 1 : 0;
+#pragma unroll
+            for (int k = W; k > 0; k--) {
+                v[k] = k > p ? v[k - 1] : v[k];
+                if (IDX) ix[k] = k > p ? ix[k - 1] : ix[k];
+            }
+#pragma unroll
+            for (int k = 0; k <= W; k++) {
+                v[k] = k == p ? cur : v[k];
+                if (IDX) ix[k] = k == p ? index : ix[k];
+            }
+            n++;
+            last = p;
+        }
+    }
+    __device__ __forceinline__ T at(int i) const {
+        T r = T(0);
+#pragma unroll
+        for (int k = 0; k <= W; k++) r = k == i ? v[k] : r;
+        return r;
+    }
+    __device__ __forceinline__ T quantile(T q, long) const {
+        if (n != W) return qnan<T>();
+        T f = q * (T)(n - 1);
+        int i = (int)f;
+        if ((T)i == f) return at(i);
+        double lo = (double)at(i), hi = (double)at(i + 1);
+        return (T)(lo + (hi - lo) * (double)(f - (T)i));
+    }
+    __device__ __forceinline__ T rank(T cur, long) const {
+        if (n != W) return qnan<T>();
+        int lt = 0;
+#pragma unroll
+        for (int k = 0; k < W; k++) lt += v[k] < cur ? 1 : 0;
+        double r = last + 1, rmin = lt + 1;
+        r = (((r * (r + 1) / 2) - ((rmin - 1) * rmin / 2)) / (r - rmin + 1));
+        return (T)r;
+    }
+    __device__ __forceinline__ T vmin_(long) const { return n != W ? qnan<T>() : v[0]; }
+    __device__ __forceinline__ T vmax_(long) const { return n != W ? qnan<T>() : v[W - 1]; }
+    __device__ __forceinline__ T argmin(int64_t cur_idx, long) const {
+        if (n != W) return qnan<T>();
+        return (T)(int64_t)(ix[0] + n - cur_idx);
+    }
+};
+
 // `v` loaded from row `row` (clamped at 0 by the caller) is NaN when the row
 // lies before the run start -- no history before start (Ops.hpp:87-92)
 template <typename T> __device__ __forceinline__ T nan_before(T v, long row) { return row >= 0 ? v : qnan<T>(); }

@@ -174,6 +174,17 @@ const void *stream_current_device(StreamCtx *c, size_t handle);
 size_t stream_ticks(StreamCtx *c);
 std::string stream_serialize(StreamCtx *c);
 
+// ---- corrWith / aggregrate (row_reduce.hip) ------------------------------
+// device: the pointers are device addresses on `st` (no synchronisation);
+// otherwise host buffers (copied, synchronous)
+void corr_with(Executor &ex, bool device, int sts, bool rank_inputs, int n, const float *const *inputs,
+               const float *target, float *const *outs, size_t S, size_t T, size_t start, size_t L, int blk,
+               hipStream_t st);
+// outs: [n][7] (sum, min, max, first, last, count, mean; null = not wanted);
+// device outputs hold out_rows rows each
+void aggregate(Executor &ex, bool device, int n, const void *const *inputs, const void *const *labels, int dtype,
+               void *const *outs, size_t S, size_t T, size_t start, size_t L, size_t out_rows, hipStream_t st);
+
 // ---- session pools (stream_pool.hip) ------------------------------------
 struct StreamPool;
 size_t pool_bytes_per_stock(const KunModuleDesc *m);

@@ -24,8 +24,11 @@
 #include <hip/hip_runtime.h>
 #include <string.h>
 
+#include <condition_variable>
+#include <functional>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "kun_cs.hpp"
@@ -65,6 +68,76 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 } // namespace
 
+// Host threads that copy a cycle's input rows into pinned staging: a large
+// cycle moves megabytes of rows, more than one core copies in the time the
+// device takes for a cycle.
+class CopyWorkers {
+  public:
+    explicit CopyWorkers(int n) {
+        for (int i = 0; i < n; i++) threads_.emplace_back([this] { loop(); });
+    }
+    ~CopyWorkers() {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : threads_) t.join();
+    }
+    // runs fn(i) for i in [0, n), the calling thread included
+    void run(size_t n, const std::function<void(size_t)> &fn) {
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            fn_ = &fn;
+            n_ = n;
+            next_ = 0;
+            active_ = (int)threads_.size();
+            gen_++;
+        }
+        cv_.notify_all();
+        work();
+        std::unique_lock<std::mutex> g(mu_);
+        done_cv_.wait(g, [this] { return active_ == 0; });
+        fn_ = nullptr;
+    }
+    int size() const { return (int)threads_.size(); }
+
+  private:
+    void work() {
+        for (;;) {
+            size_t i;
+            {
+                std::lock_guard<std::mutex> g(mu_);
+                if (next_ >= n_) return;
+                i = next_++;
+            }
+            (*fn_)(i);
+        }
+    }
+    void loop() {
+        unsigned long seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> g(mu_);
+                cv_.wait(g, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+            }
+            work();
+            std::lock_guard<std::mutex> g(mu_);
+            if (--active_ == 0) done_cv_.notify_all();
+        }
+    }
+    std::vector<std::thread> threads_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_cv_;
+    const std::function<void(size_t)> *fn_ = nullptr;
+    size_t n_ = 0, next_ = 0;
+    int active_ = 0;
+    unsigned long gen_ = 0;
+    bool stop_ = false;
+};
+
 struct PoolSession {
     bool live = false;
     size_t s0 = 0, S = 0;
@@ -99,6 +172,7 @@ struct StreamPool {
     size_t staging = 0;
     unsigned long long next_cycle = 1;
     char *scratch = nullptr;              // admission block table
+    std::unique_ptr<CopyWorkers> copiers;
     ~StreamPool();
     void *row(int b) const { return rows + (size_t)b * cap * esz; }
 };
@@ -199,6 +273,7 @@ StreamPool *pool_create(std::shared_ptr<Executor> ex, const KunModuleDesc *m, si
     }
     HIPCHECK(hipMalloc((void **)&p->scratch, align256(max_blocks * sizeof(kun::PoolBlock))));
     p->free_[0] = p->cap / 64;
+    p->copiers.reset(new CopyWorkers(7));
     return p.release();
 }
 
@@ -340,6 +415,7 @@ unsigned long long pool_submit(StreamPool *p, int n, const int *sessions, const 
     const size_t used = off_in + nin * total * p->esz;
     if (used > p->staging) throw Error("session pool staging overflow");
     kun::PoolBlock *blk = (kun::PoolBlock *)(h + off_blocks);
+    std::vector<size_t> offs(n);
     size_t off = 0, bi = 0;
     for (int j = 0; j < n; j++) {
         const PoolSession &ss = p->sess[sessions[j]];
@@ -349,13 +425,19 @@ unsigned long long pool_submit(StreamPool *p, int n, const int *sessions, const 
             blk[bi].p0 = (long)(off + b * 64);
             blk[bi].tick = ss.tick;
         }
-        for (size_t q = 0; q < nin; q++) {
-            const void *src = inputs[(size_t)j * nin + q];
-            if (!src) throw Error("missing input row");
-            memcpy(h + off_in + (q * total + off) * p->esz, src, ss.S * p->esz);
-        }
+        for (size_t q = 0; q < nin; q++)
+            if (!inputs[(size_t)j * nin + q]) throw Error("missing input row");
+        offs[j] = off;
         off += ss.S;
     }
+    auto copy_one = [&](size_t i) {
+        const size_t j = i / nin, q = i % nin;
+        memcpy(h + off_in + (q * total + offs[j]) * p->esz, inputs[i], p->sess[sessions[j]].S * p->esz);
+    };
+    if (total * nin * p->esz >= (1u << 20))
+        p->copiers->run((size_t)n * nin, copy_one);
+    else
+        for (size_t i = 0; i < (size_t)n * nin; i++) copy_one(i);
     kun_cs::Seg *seg = (kun_cs::Seg *)(h + off_segs);
     size_t si = 0;
     for (int ph = 0; ph < m->nphase; ph++) {

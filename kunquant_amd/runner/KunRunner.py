@@ -72,6 +72,11 @@ def _load():
     sig("kunLaunchGraphBatch", i, vp, vp)
     sig("kunDestroyGraphBatch", None, vp)
     sig("kunModuleStageNumKernels", i, vp, i)
+    fpp = ctypes.POINTER(vp)
+    sig("kunCorrWith", i, vp, i, i, i, fpp, vp, fpp, sz, sz, sz, sz)
+    sig("kunCorrWithDevice", i, vp, i, i, i, fpp, vp, fpp, sz, sz, sz, sz, vp)
+    sig("kunAggregrate", i, vp, i, fpp, fpp, i, fpp, sz, sz, sz, sz)
+    sig("kunAggregrateDevice", i, vp, i, fpp, fpp, i, fpp, sz, sz, sz, sz, sz, vp)
     sig("kunStreamPoolBytesPerStock", sz, vp)
     sig("kunCreateStreamPool", vp, vp, vp, sz, i, i)
     sig("kunDestroyStreamPool", None, vp)
@@ -390,6 +395,97 @@ def runGraph(exec: Executor, mod: Module, inputs: dict, cur_time: int, length: i
     return ret
 
 
+_FLOAT_SIMD_LEN, _DOUBLE_SIMD_LEN = 8, 4
+AGGREGRATION_KINDS = ("sum", "min", "max", "first", "last", "count", "mean")
+
+
+def _check_panel(arr, name: str, layout: str, dtype, known: dict, simd_len: int):
+    """The reference's checkInput (PyBinding.cpp:84-122): TS [T][S] or STs
+    [S/simd][T][simd], all buffers of one call of one shape."""
+    if not isinstance(arr, np.ndarray):
+        arr = np.asarray(arr)
+    if layout == "STs":
+        if arr.ndim != 3:
+            raise RuntimeError("Bad STs shape at " + name)
+        if not known:
+            known.update(S=arr.shape[0], T=arr.shape[1], N=arr.shape[0] * simd_len)
+        _expect(dtype, arr, name, (known["S"], known["T"], simd_len))
+    elif layout == "TS":
+        if arr.ndim != 2:
+            raise RuntimeError("Bad TS shape at " + name)
+        if not known:
+            known.update(S=arr.shape[1] // simd_len, N=arr.shape[1])
+        known.setdefault("T", arr.shape[0])
+        _expect(dtype, arr, name, (known["T"], known["N"]))
+    else:
+        raise RuntimeError("Unknown layout at " + name)
+    return arr
+
+
+def corrWith(exec: Executor, inputs, corr_with, outs, layout: str = "TS", rank_inputs: bool = False):
+    """Per-row correlation of each input with `corr_with` over the stocks
+    (PyBinding.cpp:396-446, CorrWith.hpp:11-121), float32: outs[i][t] is the
+    correlation of row t of inputs[i] (of its rank if rank_inputs) with row
+    t of corr_with."""
+    if layout not in ("TS", "STs"):
+        raise RuntimeError("Unknown layout" + layout)
+    if len(inputs) != len(outs):
+        raise RuntimeError("number of inputs and outputs should match")
+    known: dict = {}
+    target = _check_panel(corr_with, "corr_with", layout, np.float32, known, _FLOAT_SIMD_LEN)
+    ins = [_check_panel(a, f"buffer_{j}", layout, np.float32, known, _FLOAT_SIMD_LEN) for j, a in enumerate(inputs)]
+    for o in outs:
+        _expect(np.float32, o, "", (known["T"],))
+    n = len(ins)
+    vp = ctypes.c_void_p
+    inp = (vp * max(1, n))(*[a.ctypes.data for a in ins])
+    outp = (vp * max(1, n))(*[o.ctypes.data for o in outs])
+    T = known["T"]
+    if _load().kunCorrWith(exec._h, 0 if layout == "STs" else 1, 1 if rank_inputs else 0, n, inp,
+                           target.ctypes.data, outp, known["N"], T, 0, T):
+        raise RuntimeError(_err())
+
+
+def aggregrate(exec: Executor, inputs, labels, outs):
+    """Per-stock reductions over runs of equal labels (PyBinding.cpp:448-508,
+    Aggregration.cpp:11-105): outs[i] maps kinds (sum, min, max, first,
+    last, count, mean) to [groups][S] arrays for inputs[i] ([T][S], TS)
+    grouped by labels[i] ([T])."""
+    if len(inputs) != len(labels) or len(inputs) != len(outs):
+        raise RuntimeError("number of inputs, labels and outputs should match")
+    if len(inputs) == 0:
+        return
+    dtype = np.float32 if np.asarray(inputs[0]).dtype == np.float32 else np.float64
+    simd_len = _FLOAT_SIMD_LEN if dtype == np.float32 else _DOUBLE_SIMD_LEN
+    known: dict = {}
+    n = len(inputs)
+    vp = ctypes.c_void_p
+    inp, labp = (vp * n)(), (vp * n)()
+    outp = (vp * (7 * n))()
+    keep = []
+    T = 0
+    for j in range(n):
+        a = _check_panel(inputs[j], f"buffer_{j}", "TS", dtype, known, simd_len)
+        T = known["T"]
+        lab = np.asarray(labels[j])
+        _expect(dtype, lab, "label", (T,))
+        keep += [a, lab]
+        inp[j], labp[j] = a.ctypes.data, lab.ctypes.data
+        groups = 1 + int(np.count_nonzero(lab[1:] != lab[:-1])) if T else 0
+        known_out: dict = {"S": known["S"], "N": known["N"]}
+        for name, arr in outs[j].items():
+            if name not in AGGREGRATION_KINDS:
+                raise RuntimeError("Unknown aggregration kind: " + name)
+            k = AGGREGRATION_KINDS.index(name)
+            arr = _check_panel(arr, f"output_{name}{k}", "TS", dtype, known_out, simd_len)
+            if arr.shape[0] < groups:
+                raise RuntimeError(f"Bad shape at output_{name}{k}: {groups} label groups need {groups} rows")
+            keep.append(arr)
+            outp[j * 7 + k] = arr.ctypes.data
+    if _load().kunAggregrate(exec._h, n, inp, labp, 1 if dtype == np.float64 else 0, outp, known["N"], T, 0, T):
+        raise RuntimeError(_err())
+
+
 def runGraphDevice(exec: Executor, mod: Module, buffers: Dict[str, int], num_stocks: int, total_time: int,
                    cur_time: int, length: int, stream: int = 0):
     """Device-resident execution: `buffers` maps every input and output name
@@ -593,24 +689,42 @@ class StreamPool:
             raise RuntimeError(_err())
         self._stocks.pop(sid, None)
 
-    def submit(self, ticks) -> int:
-        """ticks: [(session id, {input name: row ndarray [S]})]; one tick of
-        each listed session. Returns the cycle id."""
-        n = len(ticks)
-        nin = len(self._inputs)
-        sids = (ctypes.c_int * n)()
-        rows = (ctypes.c_void_p * (n * nin))()
-        keep = []
-        dtype = self._mod.dtype
-        for j, (sid, data) in enumerate(ticks):
-            sids[j] = sid
-            S = self._stocks.get(sid)
-            for q, name in enumerate(self._inputs):
+    class Rows:
+        """Checked input rows of one tick of one session, in the module's
+        input order (build once, submit many times; keeps the arrays)."""
+        __slots__ = ("sid", "arrays", "ptrs")
+
+        def __init__(self, pool: "StreamPool", sid: int, data: Dict[str, np.ndarray]):
+            S = pool._stocks.get(sid)
+            dtype = pool._mod.dtype
+            self.sid = sid
+            self.arrays = []
+            for name in pool._inputs:
                 a = data[name]
                 if not isinstance(a, np.ndarray) or a.dtype != dtype or not a.flags.c_contiguous or a.shape != (S,):
                     raise RuntimeError(f"Bad input row {name} for session {sid}")
-                keep.append(a)
-                rows[j * nin + q] = a.ctypes.data
+                self.arrays.append(a)
+            self.ptrs = tuple(a.ctypes.data for a in self.arrays)
+
+    def rows(self, sid: int, data: Dict[str, np.ndarray]) -> "StreamPool.Rows":
+        return StreamPool.Rows(self, sid, data)
+
+    def submit(self, ticks) -> int:
+        """ticks: [(session id, {input name: row ndarray [S]})] or
+        [StreamPool.Rows]; one tick of each listed session (the rows are
+        copied before the call returns). Returns the cycle id."""
+        n = len(ticks)
+        nin = len(self._inputs)
+        sids = (ctypes.c_int * n)()
+        flat = []
+        for j, t in enumerate(ticks):
+            if not isinstance(t, StreamPool.Rows):
+                t = StreamPool.Rows(self, t[0], t[1])
+            if t.sid not in self._stocks:
+                raise RuntimeError(f"no such session {t.sid}")
+            sids[j] = t.sid
+            flat.extend(t.ptrs)
+        rows = (ctypes.c_void_p * (n * nin))(*flat)
         cyc = ctypes.c_ulonglong(0)
         if _load().kunStreamPoolSubmit(self._h, n, sids, rows, ctypes.byref(cyc)):
             raise RuntimeError(_err())

@@ -31,6 +31,7 @@ class TickServer:
         self.latencies: List[float] = []
         self.cycles = 0
         self.ticks_in_cycles = 0
+        self.submit_s = 0.0         # host time inside pool.submit
 
     # ---- sessions -------------------------------------------------------
     def open(self, key: Hashable, num_stocks: int) -> bool:
@@ -63,7 +64,8 @@ class TickServer:
 
     # ---- ticks ----------------------------------------------------------
     def push(self, key: Hashable, rows, t_arrival: Optional[float] = None):
-        """rows: {input name: row [S]}; kept by reference until dispatched."""
+        """rows: {input name: row [S]} or StreamPool.Rows of the session;
+        kept by reference until dispatched."""
         q = self.queue[key]
         q.append((self.clock() if t_arrival is None else t_arrival, rows))
         if key in self.sid and key not in self.busy:
@@ -92,11 +94,13 @@ class TickServer:
         ticks, members = [], []
         for key in self.ready:
             t0, rows = self.queue[key].popleft()
-            ticks.append((self.sid[key], rows))
+            ticks.append(rows if isinstance(rows, kr.StreamPool.Rows) else (self.sid[key], rows))
             members.append((key, t0))
             self.busy.add(key)
         self.ready.clear()
+        t = self.clock()
         cyc = self.pool.submit(ticks)
+        self.submit_s += self.clock() - t
         self.inflight.append((cyc, members))
         self.cycles += 1
         self.ticks_in_cycles += len(members)

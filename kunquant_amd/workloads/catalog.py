@@ -6,7 +6,8 @@ import os
 from typing import Callable, Dict, Tuple
 
 from .. import PACKAGE_DIR
-from .graphs import aligned_graph, cross_section_graph, ops_graph, random_graph, simple_graph, stream_graph
+from .graphs import (aligned_graph, cross_section_graph, ops_graph, precision_graph, random_graph, simple_graph,
+                     stream_graph)
 
 MODULE_DIR = os.path.join(PACKAGE_DIR, "lib", "modules")
 
@@ -36,6 +37,13 @@ CATALOG: Dict[str, Tuple[Callable, dict]] = {
     "stream_f64": (stream_graph, dict(dtype="double", input_layout="STREAM", output_layout="STREAM")),
     "stream_f32": (stream_graph, dict(input_layout="STREAM", output_layout="STREAM",
                                       options={"opt_reduce": False})),
+    # precision run: 150 factors; reference fp32 semantics, fp32 with fp64
+    # finalisation, and the fp64 reference of the same algorithm
+    "prec_f32": (lambda: precision_graph(150, 3), dict(input_layout="TS", output_layout="TS")),
+    "prec_f32x": (lambda: precision_graph(150, 3), dict(input_layout="TS", output_layout="TS",
+                                                        options={"fp64_finalize": True})),
+    "prec_f64": (lambda: precision_graph(150, 3), dict(dtype="double", input_layout="TS", output_layout="TS",
+                                                       options={"no_fast_stat": True})),
     # serving workload: the 100-factor random graph in stream mode (one row per tick)
     "serve_f32": (lambda: random_graph(100, 5000), dict(input_layout="STREAM", output_layout="STREAM",
                                                         options={"opt_reduce": False})),
@@ -63,18 +71,27 @@ def load(name: str):
 
 
 def build_all(jobs: int = 8, names=None):
-    """Compile every catalog module (several hipcc processes at a time)."""
+    """Compile every catalog module: large modules are split into several
+    translation units, and all units of all modules share `jobs` hipcc
+    processes (largest first), then each module links."""
     from concurrent.futures import ThreadPoolExecutor
-    from ..build import compile_module_sources
-    from ..codegen.hip import emit_module
+    from ..build import compile_unit, link_module
+    from ..codegen.hip import emit_module_parts
     os.makedirs(MODULE_DIR, exist_ok=True)
     todo = list(names or CATALOG)
-    srcs = {name: emit_module(module_ir(name)) for name in todo}
-
-    def one(name):
-        return compile_module_sources([srcs[name]], library_path(name), os.path.join(MODULE_DIR, "_src", name),
-                                      jobs=1)
-
-    # largest first, so the long compiles overlap the short ones
+    srcs = {}
+    for name in todo:
+        ir = module_ir(name)
+        whole = emit_module_parts(ir, 1)[0]
+        nparts = max(1, min(jobs, len(whole) // (2 << 20)))
+        srcs[name] = emit_module_parts(ir, nparts + 1) if nparts > 1 else [whole]
+    units = [(name, i, src) for name in todo for i, src in enumerate(srcs[name])]
+    workdir = lambda name: os.path.join(MODULE_DIR, "_src", name)
     with ThreadPoolExecutor(max_workers=max(1, jobs)) as pool:
-        list(pool.map(one, sorted(todo, key=lambda n: -len(srcs[n]))))
+        objs = list(pool.map(lambda u: (u[0], u[1], compile_unit(u[2], workdir(u[0]), u[1])),
+                             sorted(units, key=lambda u: -len(u[2]))))
+    by_mod = {}
+    for name, i, obj in objs:
+        by_mod.setdefault(name, {})[i] = obj
+    for name in todo:
+        link_module([by_mod[name][i] for i in sorted(by_mod[name])], library_path(name))

@@ -12,7 +12,11 @@ random_graph(n, seed): n seeded random expression trees of depth 3-8 over
 open/high/low/close/volume/amount with elementwise arithmetic,
 compare + Select, Delay/Delta, WindowedSum/Avg/Stddev/Corr/Cov,
 TsArgMax/TsRank and DecayLinear (windows from {5, 10, 20, 30, 60}); about
-30% of the internal nodes are cross-sectional Rank or Scale."""
+30% of the internal nodes are cross-sectional Rank or Scale.
+
+precision_graph(n, seed): the same trees plus WindowedQuantile, windowed
+linear-regression slope / R-squared and 60-row variance / correlation /
+covariance, 10% cross-sectional nodes."""
 import random
 import types
 
@@ -116,6 +120,49 @@ class _TreeGen:
         b = self.tree(r.randint(0, depth - 1))
         op = K.WindowedCorrelation if r.random() < 0.5 else K.WindowedCovariance
         return op(a, self.window(), b)
+
+
+class _PrecisionTreeGen(_TreeGen):
+    """Trees of the precision workload: the random-graph op mix plus
+    WindowedQuantile (q = 0.2, 0.8), windowed linear-regression slope and
+    R-squared, and long-window (60) variance and correlation, with fewer
+    cross-sectional nodes."""
+
+    def tree(self, depth: int):
+        K, r = self.K, self.rng
+        if depth <= 0:
+            return self.leaf()
+        if r.random() < 0.3:
+            self.counts["nodes"] += 1
+            self.counts["windowed"] += 1
+            a = self.tree(depth - 1)
+            kind = r.randrange(6)
+            if kind == 0:
+                return K.WindowedQuantile(a, self.window(), r.choice((0.2, 0.8)))
+            if kind == 1:
+                return K.WindowedLinearRegressionSlope(a, self.window())
+            if kind == 2:
+                return K.WindowedLinearRegressionRSqaure(a, self.window())
+            if kind == 3:
+                return K.WindowedStddev(a, 60)
+            if kind == 4:
+                return K.WindowedCorrelation(a, 60, self.tree(r.randint(0, depth - 1)))
+            return K.WindowedCovariance(a, 60, self.tree(r.randint(0, depth - 1)))
+        return super().tree(depth)
+
+
+def precision_graph(num_factors: int, seed: int, prefix: str = "p", K=None):
+    """Seeded factor trees of depth 3-8 for the precision run (see
+    _PrecisionTreeGen); inputs are the OHLCV fields."""
+    K = K or native()
+    gen = _PrecisionTreeGen(K, seed, cs_frac=0.1)
+    b = K.Builder()
+    with b:
+        for i in range(num_factors):
+            K.Output(gen.tree(gen.rng.randint(3, 8)), f"{prefix}{i:03d}")
+    f = K.Function(b.ops)
+    f.workload_stats = dict(gen.counts)
+    return f
 
 
 def random_graph(num_factors: int, seed: int, prefix: str = "f", K=None):

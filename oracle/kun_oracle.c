@@ -430,3 +430,41 @@ void kun_oracle_stream_free(void *sp) {
     for (int b = 0; b < pg->nbuf; b++) free(st->rows[b]);
     free(st->rows); free(st->ctx); free(st->views); free(st);
 }
+
+/* ---- corrWith / rank corrWith (float), CorrWith.hpp:11-121 ------------
+ * Per row t in [start, start + L): sums over the stocks in stock order of
+ * x, y, x*x, y*y, x*y (no NaN skipping), each divided by S, then
+ * (XY - X*Y) / (sqrt(X2 - X*X) * sqrt(Y2 - Y*Y)) into out[t - start]. With
+ * rank_inputs, x is first the row's rank (RankStocks' formula, NaN stays
+ * NaN). Layout of x and y: TS [T][S], or STs [S/blk][T][blk]. */
+void kun_oracle_corrwith(const float *x, const float *y, float *out, long S, long T, long start, long L, int sts,
+                         int blk, int rank_inputs) {
+    float *row = (float *)malloc(sizeof(float) * S * 3);
+    float *rk = row + S, *tmp = row + 2 * S;
+    for (long t = start; t < start + L; t++) {
+        for (long s = 0; s < S; s++) row[s] = x[sts ? ((s / blk) * T + t) * blk + s % blk : t * S + s];
+        const float *xv = row;
+        if (rank_inputs) {
+            rank_row_f32(row, rk, S, tmp);
+            xv = rk;
+        }
+        float XY = 0, X = 0, Y = 0, X2 = 0, Y2 = 0;
+        for (long s = 0; s < S; s++) {
+            float a = xv[s];
+            float b = y[sts ? ((s / blk) * T + t) * blk + s % blk : t * S + s];
+            X += a;
+            Y += b;
+            X2 += a * a;
+            Y2 += b * b;
+            XY += a * b;
+        }
+        float n = (float)S;
+        XY /= n;
+        X /= n;
+        Y /= n;
+        X2 /= n;
+        Y2 /= n;
+        out[t - start] = (XY - X * Y) / (sqrtf(X2 - X * X) * sqrtf(Y2 - Y * Y));
+    }
+    free(row);
+}

@@ -45,6 +45,12 @@ def lib():
         L.kun_oracle_stream_row.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.kun_oracle_stream_run.argtypes = [ctypes.c_void_p]
         L.kun_oracle_stream_free.argtypes = [ctypes.c_void_p]
+        vp = ctypes.c_void_p
+        L.kun_oracle_corrwith.argtypes = [vp, vp, vp, ctypes.c_long, ctypes.c_long, ctypes.c_long, ctypes.c_long,
+                                          ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        for sfx in ("f32", "f64"):
+            getattr(L, f"kun_oracle_aggregate_{sfx}").argtypes = [vp, vp, ctypes.POINTER(vp), ctypes.c_long,
+                                                                  ctypes.c_long, ctypes.c_long, ctypes.c_long]
         _lib = L
     return _lib
 
@@ -144,3 +150,39 @@ class OracleStream:
 
     def get(self, name: str) -> np.ndarray:
         return self._row(name).copy()
+
+
+AGG_KINDS = ("sum", "min", "max", "first", "last", "count", "mean")
+
+
+def corr_with(x: np.ndarray, y: np.ndarray, layout: str = "TS", rank_inputs: bool = False, blocking: int = 8,
+              start: int = 0, length: Optional[int] = None) -> np.ndarray:
+    """Per-row correlation of x with y over the stocks (float32)."""
+    if layout == "TS":
+        T, S = x.shape
+    else:
+        nb, T, blk = x.shape
+        S, blocking = nb * blk, blk
+    L = T - start if length is None else length
+    out = np.empty(L, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    lib().kun_oracle_corrwith(x.ctypes.data, y.ctypes.data, out.ctypes.data, S, T, start, L,
+                              1 if layout == "STs" else 0, blocking, 1 if rank_inputs else 0)
+    return out
+
+
+def aggregate(x: np.ndarray, labels: np.ndarray, start: int = 0, length: Optional[int] = None):
+    """{kind: [groups][S]} of the rows of x ([T][S]) grouped by runs of equal
+    labels."""
+    T, S = x.shape
+    L = T - start if length is None else length
+    lab = labels[start:start + L]
+    groups = 1 + int(np.count_nonzero(lab[1:] != lab[:-1])) if L > 0 else 0
+    outs = {k: np.empty((groups, S), dtype=x.dtype) for k in AGG_KINDS}
+    ptrs = (ctypes.c_void_p * 7)(*[outs[k].ctypes.data for k in AGG_KINDS])
+    x = np.ascontiguousarray(x)
+    labels = np.ascontiguousarray(labels, dtype=x.dtype)
+    fn = lib().kun_oracle_aggregate_f64 if x.dtype == np.float64 else lib().kun_oracle_aggregate_f32
+    fn(x.ctypes.data, labels.ctypes.data, ptrs, S, T, start, L)
+    return outs

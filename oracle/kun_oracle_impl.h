@@ -631,3 +631,42 @@ static void FN(scale_row)(const REAL *in, REAL *out, long S) {
     for (long s = 0; s < S; s++) if (in[s] == in[s]) sum += FN(from_bits)(FN(bits)(in[s]) & ~((UBITS)1 << (sizeof(UBITS) * 8 - 1)));
     for (long s = 0; s < S; s++) out[s] = (in[s] == 0 && sum == 0) ? (REAL)NAN : in[s] / sum;
 }
+
+/* ---- aggregrate: per stock, rows split into groups where the label
+ * changes; per group sum (kahanAdd), min, max (NaN if any NaN), first,
+ * last, count (every row), mean = sum / count -- Aggregration.cpp:11-105.
+ * outs[k] for k = sum, min, max, first, last, count, mean (NULL: not
+ * wanted) are [groups][S]; rows start .. start + L - 1 of in ([T][S]) and
+ * labels ([T]) are read. */
+void FN(kun_oracle_aggregate)(const REAL *in, const REAL *labels, REAL *const *outs, long S, long T, long start,
+                              long L) {
+    (void)T;
+    if (L <= 0) return;
+    for (long s = 0; s < S; s++) {
+        REAL sum = 0, comp = 0, mn = INFINITY, mx = -INFINITY, first = in[start * S + s], last = 0, count = 0;
+        REAL last_label = labels[start];
+        long g = 0;
+        for (long i = 0; i < L; i++) {
+            REAL label = labels[start + i];
+            REAL cur = in[(start + i) * S + s];
+            if (label != last_label) {
+                REAL vals[7] = {sum, mn, mx, first, last, count, sum / count};
+                for (int k = 0; k < 7; k++) if (outs[k]) outs[k][g * S + s] = vals[k];
+                g++;
+                first = cur;
+                sum = 0; comp = 0; mn = INFINITY; mx = -INFINITY; count = 0;
+            }
+            count = count + (REAL)1;
+            last = cur;
+            last_label = label;
+            mx = (mx != mx || cur != cur) ? (REAL)NAN : (mx > cur ? mx : cur);
+            mn = (mn != mn || cur != cur) ? (REAL)NAN : (mn < cur ? mn : cur);
+            REAL y = cur - comp;
+            REAL t = sum + y;
+            comp = (t - sum) - y;
+            sum = t;
+        }
+        REAL vals[7] = {sum, mn, mx, first, last, count, sum / count};
+        for (int k = 0; k < 7; k++) if (outs[k]) outs[k][g * S + s] = vals[k];
+    }
+}

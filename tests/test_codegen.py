@@ -132,3 +132,58 @@ def test_plan_levels_of_running_sum():
     plan = stage_plan(m, st)
     assert [lv.kind for lv in plan.levels] == [SEQ, PAR]
     assert len(plan.mat) == 1
+
+
+def test_fp64_finalize_emission():
+    """Option fp64_finalize: window-loop sums and regression sums are double
+    accumulators rounded to float once; without it every accumulator is
+    float; stream modules reject it."""
+    from kunquant_amd.ops import WindowedLinearRegressionSlope, WindowedStddev
+
+    def graph():
+        b = Builder()
+        with b:
+            a = Input("a")
+            Output(WindowedStddev(a, 20), "std")
+            Output(WindowedLinearRegressionSlope(a, 10), "slope")
+        return Function(b.ops)
+
+    src = emit_module(compileit(graph(), "fin", input_layout="TS", output_layout="TS",
+                                options={"fp64_finalize": True}))
+    assert "kun::LinReg<double>" in src and "double r" in src
+    plain = emit_module(compileit(graph(), "nofin", input_layout="TS", output_layout="TS"))
+    assert "kun::LinReg<float>" in plain and "double r" not in plain
+    with pytest.raises(RuntimeError, match="fp64_finalize"):
+        emit_module(compileit(graph(), "sfin", input_layout="STREAM", output_layout="STREAM",
+                              options={"fp64_finalize": True, "opt_reduce": False}))
+
+
+def test_stream_phase_plan():
+    """Pool launch plan of the 100-factor stream module: every stage in
+    exactly one phase; phases alternate time-series / cross-sectional; a
+    time-series stage's producers run in an earlier phase or, if they are
+    time-series, earlier in the same phase; a cross-sectional stage's
+    producers run in earlier phases."""
+    from kunquant_amd.codegen.hip import STAGE_TS, stream_phases
+    ir = catalog.module_ir("serve_f32")
+    phases = stream_phases(ir)
+    where = {}
+    for p, (kind, sts) in enumerate(phases):
+        assert kind == p % 2 or all((st.kind == STAGE_TS) == (kind == 0) for st in sts)
+        for j, st in enumerate(sts):
+            assert st.idx not in where
+            where[st.idx] = (p, j)
+            assert (st.kind == STAGE_TS) == (kind == 0)
+    assert sorted(where) == sorted(st.idx for st in ir.stages)
+    writer = {b.idx: st for st in ir.stages for b in st.out_bufs}
+    for st in ir.stages:
+        p, j = where[st.idx]
+        for b in st.in_bufs:
+            w = writer.get(b.idx)
+            if w is None or w is st:
+                continue
+            pw, jw = where[w.idx]
+            if st.kind == STAGE_TS and w.kind == STAGE_TS:
+                assert (pw, jw) < (p, j)
+            else:
+                assert pw < p

@@ -402,3 +402,48 @@ def test_stream_equals_batch():
         st.run()
         for k in ("std", "tsr", "rk"):
             np.testing.assert_array_equal(st.get(k), batch[k][i])
+
+
+# ---- corrWith / aggregrate (test_runtime.py:17-122) ------------------------
+
+def test_oracle_corrwith_matches_pandas():
+    """CorrWith.hpp against pandas corrwith(axis=1), plain and on ranks, at
+    the reference test's tolerance; TS and STs layouts agree exactly."""
+    import pandas as pd
+    from oracle import kun_oracle as ko
+    rng = np.random.default_rng(41)
+    a = rng.random((24, 20)).astype("float32")
+    r = rng.random((24, 20)).astype("float32")
+    np.testing.assert_allclose(ko.corr_with(a, r), pd.DataFrame(a).corrwith(pd.DataFrame(r), axis=1),
+                               atol=1e-6, rtol=1e-4)
+    a2 = pd.DataFrame(a).rank(axis=1, pct=True).astype("float32")
+    np.testing.assert_allclose(ko.corr_with(a, r, rank_inputs=True), a2.corrwith(pd.DataFrame(r), axis=1),
+                               atol=1e-6, rtol=1e-4)
+    sts = lambda x: np.ascontiguousarray(x.reshape(24, 20 // 4, 4).transpose(1, 0, 2))
+    assert np.array_equal(ko.corr_with(sts(a), sts(r), layout="STs"), ko.corr_with(a, r), equal_nan=True)
+    # a NaN cell makes its row NaN (no NaN skipping in the sums)
+    a[3, 7] = np.nan
+    out = ko.corr_with(a, r)
+    assert np.isnan(out[3]) and not np.isnan(out[4])
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_oracle_aggregate_matches_pandas(dtype):
+    """Aggregration.cpp against pandas groupby(...).sum/min/max/first/last/
+    count/mean over runs of equal labels (the reference test's data shape)."""
+    import pandas as pd
+    from oracle import kun_oracle as ko
+    rng = np.random.default_rng(42)
+    a = rng.random((240, 16)).astype(dtype)
+    dates = pd.date_range(start="2026-01-01", end="2026-01-07", freq="min")
+    stamps = pd.DatetimeIndex(sorted(rng.choice(dates, size=240)))
+    df = pd.DataFrame(a, index=stamps)
+    g = df.groupby(df.index.date)
+    labels = stamps.day.to_numpy().astype(dtype)
+    got = ko.aggregate(a, labels)
+    for k in ko.AGG_KINDS:
+        np.testing.assert_allclose(got[k], getattr(g, k)(), atol=1e-6, rtol=1e-4, err_msg=k)
+    # start offset: the groups of rows 30.. only
+    sub = ko.aggregate(a, labels, start=30)
+    g2 = pd.DataFrame(a[30:], index=stamps[30:]).groupby(stamps[30:].date)
+    np.testing.assert_allclose(sub["sum"], g2.sum(), atol=1e-6, rtol=1e-4)

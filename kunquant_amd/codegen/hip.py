@@ -22,6 +22,7 @@ rings and carried state live in HBM as [slot][S] arrays owned by the stream
 context (the reference's StreamBuffer / StateBuffer, Runtime.cpp:384-546).
 Cross-sectional stages (Rank / Scale) are runtime library kernels."""
 import os
+import re
 from typing import Dict, List, Optional, Tuple
 
 from ..Driver import BUF_INPUT, BUF_TEMP, ModuleIR, STAGE_TS
@@ -39,8 +40,13 @@ LDS_BUDGET = int(os.environ.get("KUN_LDS_BUDGET", 40 * 1024))
 PREFETCH_REGS = int(os.environ.get("KUN_PREFETCH_REGS", 64))
 SEQ_PREFETCH_MAX = int(os.environ.get("KUN_SEQ_PREFETCH_MAX", 16))
 # unroll factor of window loops
-LOOP_UNROLL = int(os.environ.get("KUN_LOOP_UNROLL", 4))
+LOOP_UNROLL = int(os.environ.get("KUN_LOOP_UNROLL", 8))
 SORTED_REGS = 256  # register budget (values) for the sorted windows of one sequential component
+# row-chunked components: rows unrolled per group by estimated cost per row
+# (first threshold met wins; lighter components unroll 8 rows)
+PAR_UNROLL_BY_COST = ((1024, 1), (256, 2), (64, 4))
+# row-chunked components with window loops: rows per interleaved group
+INTERLEAVE_MIN, INTERLEAVE_MAX = 2, 4
 
 _BIN = {"Add": "+", "Sub": "-", "Mul": "*", "Div": "/"}
 _CMP = {"GreaterThan": ">", "GreaterEqual": ">=", "LessThan": "<", "LessEqual": "<=", "Equals": "=="}
@@ -94,6 +100,79 @@ def _window_reads(op) -> Optional[Tuple[OpBase, int]]:
     if name in _LAG_OPS:
         return op.inputs[0], op.attrs["window"]
     return None
+
+
+_DEF = re.compile(r"\bconst\s+\w+\s+(\w+)\s*=")
+_ITER = re.compile(r"\biter\b")
+
+
+def loop_code(w: int, body: List[str], unroll: int) -> List[str]:
+    """A window loop over iter = w-1 .. 0 (oldest value first), unrolled by
+    hand in groups of `unroll` iterations (0: the whole window): the group's
+    window reads -- statements that depend on nothing but `iter` -- are
+    issued first, then the iterations run in order. Every statement of the
+    loop body keeps its order relative to the others of its iteration, so
+    the arithmetic is that of the plain loop; only the loads move earlier,
+    so that their latency overlaps the dependent chains."""
+    U = w if unroll <= 0 else min(unroll, w)
+    defined = set()
+    for x in body:
+        defined.update(_DEF.findall(x))
+    hoist = [i for i, x in enumerate(body)
+             if x.lstrip().startswith("const ") and _ITER.search(x) and not any(
+                 re.search(rf"\b{n}\b", x.split("=", 1)[1]) for n in defined)]
+
+    def copy(u: int) -> List[str]:
+        out = []
+        for x in body:
+            x = _ITER.sub(f"(it0 - {u})", x)
+            for n in defined:
+                x = re.sub(rf"\b{n}\b", f"{n}u{u}", x)
+            out.append(x)
+        return out
+
+    L = ["{", "int it0 = " + str(w - 1) + ";", f"for (; it0 >= {U - 1}; it0 -= {U}) {{"]
+    copies = [copy(u) for u in range(U)]
+    for c in copies:
+        L.extend(c[i] for i in hoist)
+    for c in copies:
+        L.extend(c[i] for i in range(len(c)) if i not in hoist)
+    L.append("}")
+    if w % U:
+        L.append("for (int iter = it0; iter >= 0; --iter) {")
+        L.extend(body)
+        L.append("}")
+    L.append("}")
+    return L
+
+
+def _flatten(segs, unroll: int) -> List[str]:
+    L = []
+    for sg in segs:
+        if sg[0] == "lines":
+            L.extend(sg[1])
+        else:
+            L.extend(sg[1] + loop_code(sg[2], sg[3], unroll) + sg[4])
+    return L
+
+
+_RENAME = [(re.compile(r"\br(\d+)_"), r"r\1q{q}_"), (re.compile(r"\blg(\d+)_(\d+)\b"), r"lg\1_\2q{q}"),
+           (re.compile(r"\brp(\d+)\b"), r"rp\1q{q}"), (re.compile(r"\bi\b"), "iq{q}")]
+_VNAME = re.compile(r"\bv(\d+)\b")
+
+
+def _rename_row(lines: List[str], q: int, keep: set) -> List[str]:
+    """Names of one row's statements made private to row copy q of an
+    interleaved group: values v<k> (but those in `keep`, declared outside
+    the row), reduction state r<k>_*, lag values, ring positions and the row
+    index i."""
+    out = []
+    for x in lines:
+        x = _VNAME.sub(lambda m: m.group(0) if int(m.group(1)) in keep else f"v{m.group(1)}_{q}", x)
+        for pat, rep in _RENAME:
+            x = pat.sub(rep.replace("{q}", str(q)), x)
+        out.append(x)
+    return out
 
 
 class _Lowering:
@@ -248,6 +327,12 @@ class _Lowering:
         return L
 
     def loop_lines(self, loop) -> List[str]:
+        pre, w, body, post = self.loop_parts(loop)
+        return pre + loop_code(w, body, self.loop_unroll) + post
+
+    def loop_parts(self, loop):
+        """(declarations, window, loop body, finalisation) of a window loop
+        (see loop_code)."""
         T = self.T
         w = loop.attrs["window"]
         body = [op for op in self.ops if op.get_parent() is loop]
@@ -277,9 +362,8 @@ class _Lowering:
                 L.append(f"{T} r{k}_w = r{k}_step, r{k}_v = 0;")
             else:
                 raise RuntimeError(f"HIP emitter: unsupported reduction {name}")
-        L.append("{")
-        L.append(f"#pragma unroll {self.loop_unroll}" if self.loop_unroll else "#pragma unroll")
-        L.append(f"for (int iter = {w - 1}; iter >= 0; --iter) {{")
+        pre = L
+        L = []
         for op in body:
             L.extend("  " + x for x in self.op_lines(op))
         for r in reds:
@@ -308,8 +392,8 @@ class _Lowering:
                          f" r{k}_eq = ({x} == r{k}_cur) ? r{k}_eq + {T}(1.0) : r{k}_eq; }}")
             elif name == "ReduceDecayLinear":  # Ops.hpp:507-520
                 L.append(f"  r{k}_v = kun::fma({x}, r{k}_w, r{k}_v); r{k}_w = r{k}_w + r{k}_step;")
-        L.append("}")
-        L.append("}")
+        body_lines = L
+        L = []
         for r in reds:
             k = self.idx[r]
             name = r.__class__.__name__
@@ -322,15 +406,23 @@ class _Lowering:
             elif name == "ReduceRank":
                 L.append(f"v{k} = r{k}_lt + (r{k}_eq + {T}(1.0)) / {T}(2.0);")
             L.extend(self.after_value(k))
-        return L
+        return pre, w, body_lines, L
 
-    def step_lines(self, skip=()) -> List[str]:
-        L = []
+    def step_segments(self, skip=()) -> list:
+        """Per top-level op: ("lines", statements) or ("loop", declarations,
+        header, body, finalisation)."""
+        segs = []
         for op in self.ops:
             if op.get_parent() is not None or isinstance(op, ReductionOp) or op in skip:
                 continue
-            L.extend(self.loop_lines(op) if isinstance(op, ForeachBackWindow) else self.op_lines(op))
-        return L
+            if isinstance(op, ForeachBackWindow):
+                segs.append(("loop",) + tuple(self.loop_parts(op)))
+            else:
+                segs.append(("lines", self.op_lines(op)))
+        return segs
+
+    def step_lines(self, skip=()) -> List[str]:
+        return _flatten(self.step_segments(skip), self.loop_unroll)
 
     def buffer_ptrs(self, args: str = "A") -> List[str]:
         T = self.T
@@ -564,6 +656,18 @@ class CompEmitter(_Lowering):
             else:
                 assert src in opset, "window read of a value outside the component"
                 ring_need[src] = max(ring_need.get(src, 0), off)
+        # Row-chunked components with window loops run their rows in
+        # interleaved groups of P (see group_lines); P is fixed here because
+        # it sizes the rings.
+        self.cost = self._cost()
+        has_loop = any(isinstance(op, ForeachBackWindow) for op in self.ops)
+        self.interleave = 0
+        if kind == PAR and has_loop:
+            p = next((pp for c, pp in PAR_UNROLL_BY_COST if self.cost >= c), 8)
+            p = min(max(p, INTERLEAVE_MIN), INTERLEAVE_MAX)
+            p = max(1, min(p, PREFETCH_REGS // max(1, len(self.srcs) + len(self.lags))))
+            self.interleave = p if p > 1 else 0
+        extra = self.interleave - 1 if self.interleave else 0
         self.ring: Dict[int, Tuple[int, int]] = {}
         self.tslot: Dict[int, int] = {}
         self.islot: Dict[int, int] = {}
@@ -571,7 +675,7 @@ class CompEmitter(_Lowering):
         n_lds = 0
         spilled = []
         for op in sorted(ring_need, key=lambda o: self.idx[o]):
-            slots = ring_need[op] + 1
+            slots = ring_need[op] + 1 + extra
             if n_lds + slots <= cap:
                 self.ring[self.idx[op]] = (slots, n_lds)
                 n_lds += slots
@@ -580,7 +684,7 @@ class CompEmitter(_Lowering):
         self.src_ring: Dict[int, int] = {}  # row source idx -> largest offset, ringed in LDS
         self.direct: set = set()            # row sources read by IterValue straight from HBM
         for op in sorted(src_iter, key=lambda o: (-src_iter[o], self.idx[o])):
-            slots = src_iter[op] + 1
+            slots = src_iter[op] + 1 + extra
             if n_lds + slots <= cap:
                 self.ring[self.idx[op]] = (slots, n_lds)
                 self.src_ring[self.idx[op]] = src_iter[op]
@@ -614,10 +718,8 @@ class CompEmitter(_Lowering):
                 n_i += w + 1
         self.n_t, self.n_i = n_t, n_i
         self.halo = self._halo() if kind == PAR else 0
-        self.cost = self._cost()
         self.emit_stores = True
         nops = len(self.computed)
-        has_loop = any(isinstance(op, ForeachBackWindow) for op in self.ops)
         # Row loops are unrolled P times; copy q owns prefetch slot q, which
         # always holds the next row r with r % P == q (loaded P rows ahead),
         # so loads stay in flight across P rows of work. Light row bodies
@@ -625,12 +727,14 @@ class CompEmitter(_Lowering):
         self.depth = 1
         nload = max(1, sum(1 for op in self.srcs if self._src_row_used(op)) + len(self.lags))
         if kind == PAR:
-            p = 1 if self.cost >= 1024 else 2 if self.cost >= 256 else 4 if self.cost >= 64 else 8
+            p = next((pp for c, pp in PAR_UNROLL_BY_COST if self.cost >= c), 8)
         elif has_loop or nops > 60:
             p = 1
         else:
             p = SEQ_PREFETCH_MAX if nops <= 24 else 4
         self.prefetch = max(1, min(p, PREFETCH_REGS // nload))
+        if self.interleave:
+            self.prefetch = self.interleave
 
     # -- planning helpers -------------------------------------------------
     def halo_prefill_rows(self) -> int:
@@ -810,6 +914,10 @@ class CompEmitter(_Lowering):
         """Statements of one row `i`. With the row body unrolled P > 1 times,
         prefetch slot q holds this row and is refilled P rows ahead; with
         P == 1 the slots form a shift register `depth` rows deep."""
+        return _flatten(self.row_segments(q, P), self.loop_unroll)
+
+    def row_segments(self, q: int, P: int) -> list:
+        """row_lines as segments (see step_segments)."""
         T = self.T
         L = []
         D = P if P > 1 else self.depth
@@ -832,7 +940,31 @@ class CompEmitter(_Lowering):
         for (k, off) in self.lags:
             op = self.st.func.ops[k]
             L.append(f"const {T} lg{k}_{off} = " + take(f"pl{k}_{off}_", self.lag_load(op, f"{nxt} - {off}")))
-        L.extend(self.step_lines(skip=set(self.srcs) | self.single))
+        return [("lines", L)] + self.step_segments(skip=set(self.srcs) | self.single)
+
+    def group_lines(self, P: int) -> List[str]:
+        """Rows i0 .. i0 + P - 1 interleaved op by op: each op is emitted for
+        the P rows in turn, and a window loop runs once with the P rows'
+        bodies side by side, so their dependent chains (Kahan sums, running
+        extremes) overlap. Each row computes exactly what it computes alone;
+        rings hold P - 1 extra slots because the later rows' pushes precede
+        the earlier rows' window reads."""
+        keep = {self.idx[op] for op in self.single}
+        segs = [self.row_segments(q, P) for q in range(P)]
+        L = [f"const long iq{q} = i0 + {q};" for q in range(P)]
+        for j, sg in enumerate(segs[0]):
+            if sg[0] == "lines":
+                for q in range(P):
+                    L.extend(_rename_row(segs[q][j][1], q, keep))
+                continue
+            for q in range(P):
+                L.extend(_rename_row(segs[q][j][1], q, keep))
+            body = []
+            for q in range(P):
+                body.extend(_rename_row(segs[q][j][3], q, keep))
+            L.extend(loop_code(sg[2], body, self.loop_unroll))
+            for q in range(P):
+                L.extend(_rename_row(segs[q][j][4], q, keep))
         return L
 
     def body(self) -> List[str]:
@@ -924,10 +1056,13 @@ class CompEmitter(_Lowering):
                 L.append("  } }")
             L.append(f"  i0 += {P};")
         L.append(f"  for (; i0 + {P} <= {hi}; i0 += {P}) {{")
-        for q in range(P):
-            L.append(f"    {{ const long i = i0 + {q};")
-            L.extend("      " + x for x in self.row_lines(q, P))
-            L.append("    }")
+        if self.interleave:
+            L.extend("    " + x for x in self.group_lines(P))
+        else:
+            for q in range(P):
+                L.append(f"    {{ const long i = i0 + {q};")
+                L.extend("      " + x for x in self.row_lines(q, P))
+                L.append("    }")
         L.append("  }")
         for q in range(P - 1):
             L.append(f"  if (i0 + {q} < {hi}) {{ const long i = i0 + {q};")

@@ -162,14 +162,19 @@ __global__ __launch_bounds__(BLOCK) void rank_segs(const kun_cs::Seg *segs) {
     rank_row<T, BLOCK, ITEMS, MERGE>(in, out, g.n, 1, 0);
 }
 
-// SCALE_ROWS rows per workgroup; the sum of each row is one sequential chain
-// (lane r of the first wave owns row r), fed from LDS tiles that the whole
+// ROWS rows per workgroup; the sum of each row is one sequential chain (lane
+// r of the first wave owns row r), fed from LDS tiles that the whole
 // workgroup loads one tile ahead and stores already masked: |x|, or +0 for a
 // NaN cell -- adding +0 to a sum of non-negative terms leaves it
 // bit-identical, so the chain is one dependent add per stock. LIN / LOUT are
 // the layouts (-1: read from the view at run time).
+//
+// Many rows (batch runs): the sums go to scratch (4 rows per workgroup, the
+// chain being the bound) and a second kernel divides every cell on all CUs;
+// one row (stream ticks, pool segments): one workgroup sums and divides.
 constexpr int SCALE_BLOCK = 256;
 constexpr int SCALE_ROWS = 8;
+constexpr int SCALE_SUM_ROWS = 4;
 constexpr int SCALE_TILE = 1024;
 
 template <int LAY> __device__ __forceinline__ size_t lay_at(const kun_cs::View &v, long row, long s, long S, int blk) {
@@ -178,13 +183,14 @@ template <int LAY> __device__ __forceinline__ size_t lay_at(const kun_cs::View &
     return cs_at(v.layout, row, s, S, v.rows, blk);
 }
 
-template <typename T, int LIN, int LOUT>
+// sums != null: store the row sums there and stop
+template <typename T, int LIN, int LOUT, int ROWS>
 __device__ __forceinline__ void scale_block(const kun_cs::View &in, const kun_cs::View &out, long S, long rows,
-                                            int blk, const long r0) {
-    constexpr int PER = SCALE_ROWS * SCALE_TILE / SCALE_BLOCK;
-    __shared__ T tile[SCALE_ROWS * (SCALE_TILE + 1)];
-    __shared__ T total[SCALE_ROWS];
-    const int nr = (int)((rows - r0) < SCALE_ROWS ? (rows - r0) : SCALE_ROWS);
+                                            int blk, const long r0, T *sums) {
+    constexpr int PER = ROWS * SCALE_TILE / SCALE_BLOCK;
+    __shared__ T tile[ROWS * (SCALE_TILE + 1)];
+    __shared__ T total[ROWS];
+    const int nr = (int)((rows - r0) < ROWS ? (rows - r0) : ROWS);
     const T *src = (const T *)in.ptr;
     T *dst = (T *)out.ptr;
     T reg[PER];
@@ -212,10 +218,23 @@ __device__ __forceinline__ void scale_block(const kun_cs::View &in, const kun_cs
         if ((int)threadIdx.x < nr) {
             const int n = (int)((S - c0) < SCALE_TILE ? (S - c0) : SCALE_TILE);
             const T *row = tile + threadIdx.x * (SCALE_TILE + 1);
-            // in stock order, as the reference's scalar loop
-#pragma unroll 16
-            for (int k = 0; k < n; k++) sum = sum + row[k];
+            // in stock order, as the reference's scalar loop; each group's
+            // LDS reads are issued before its adds, so their latency
+            // overlaps the chain
+            int k = 0;
+            for (; k + 16 <= n; k += 16) {
+                T g[16];
+#pragma unroll
+                for (int u = 0; u < 16; u++) g[u] = row[k + u];
+#pragma unroll
+                for (int u = 0; u < 16; u++) sum = sum + g[u];
+            }
+            for (; k < n; k++) sum = sum + row[k];
         }
+    }
+    if (sums) {
+        if ((int)threadIdx.x < nr) sums[r0 + threadIdx.x] = sum;
+        return;
     }
     if ((int)threadIdx.x < nr) total[threadIdx.x] = sum;
     __syncthreads();
@@ -232,18 +251,51 @@ __device__ __forceinline__ void scale_block(const kun_cs::View &in, const kun_cs
 template <typename T, int LIN, int LOUT>
 __global__ __launch_bounds__(SCALE_BLOCK) void scale_rows(kun_cs::View in, kun_cs::View out, long S, long rows,
                                                            int blk) {
-    scale_block<T, LIN, LOUT>(in, out, S, rows, blk, (long)blockIdx.x * SCALE_ROWS);
+    scale_block<T, LIN, LOUT, SCALE_ROWS>(in, out, S, rows, blk, (long)blockIdx.x * SCALE_ROWS, nullptr);
+}
+
+template <typename T, int LIN>
+__global__ __launch_bounds__(SCALE_BLOCK) void scale_sums(kun_cs::View in, long S, long rows, int blk, T *sums) {
+    scale_block<T, LIN, LIN, SCALE_SUM_ROWS>(in, in, S, rows, blk, (long)blockIdx.x * SCALE_SUM_ROWS, sums);
+}
+
+// out = x / sum of its row, or NaN where x == 0 and the sum is 0; one cell
+// per thread over rows x S
+template <typename T, int LIN, int LOUT>
+__global__ __launch_bounds__(256) void scale_div(kun_cs::View in, kun_cs::View out, long S, long rows, int blk,
+                                                 const T *sums) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= rows * S) return;
+    const long r = i / S, s = i - r * S;
+    const T x = ((const T *)in.ptr)[lay_at<LIN>(in, r + in.row0, s, S, blk)];
+    const T tot = sums[r];
+    ((T *)out.ptr)[lay_at<LOUT>(out, r + out.row0, s, S, blk)] = (x == T(0) && tot == T(0)) ? kun::qnan<T>() : x / tot;
 }
 
 template <typename T> __global__ __launch_bounds__(SCALE_BLOCK) void scale_segs(const kun_cs::Seg *segs) {
     const kun_cs::Seg g = segs[blockIdx.x];
     const kun_cs::View in{const_cast<void *>(g.in), 2, 1, 0}, out{g.out, 2, 1, 0};
-    scale_block<T, 2, 2>(in, out, g.n, 1, 1, 0);
+    scale_block<T, 2, 2, SCALE_ROWS>(in, out, g.n, 1, 1, 0, nullptr);
+}
+
+template <typename T, int LIN, int LOUT>
+hipError_t launch_scale_split(const kun_cs::View &in, const kun_cs::View &out, long S, long rows, int blk,
+                              hipStream_t st, T *sums) {
+    hipLaunchKernelGGL((scale_sums<T, LIN>), dim3((unsigned)((rows + SCALE_SUM_ROWS - 1) / SCALE_SUM_ROWS)),
+                       dim3(SCALE_BLOCK), 0, st, in, S, rows, blk, sums);
+    hipLaunchKernelGGL((scale_div<T, LIN, LOUT>), dim3((unsigned)((rows * S + 255) / 256)), dim3(256), 0, st, in, out,
+                       S, rows, blk, (const T *)sums);
+    return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_scale_t(const kun_cs::View &in, const kun_cs::View &out, long S, long rows, int blk,
-                          hipStream_t st) {
+                          hipStream_t st, void *scratch) {
+    if (scratch && rows > 1) {
+        T *sums = (T *)scratch;
+        if (in.layout == 1 && out.layout == 1) return launch_scale_split<T, 1, 1>(in, out, S, rows, blk, st, sums);
+        return launch_scale_split<T, -1, -1>(in, out, S, rows, blk, st, sums);
+    }
     const unsigned grid = (unsigned)((rows + SCALE_ROWS - 1) / SCALE_ROWS);
     if (in.layout == 1 && out.layout == 1)
         hipLaunchKernelGGL((scale_rows<T, 1, 1>), dim3(grid), dim3(SCALE_BLOCK), 0, st, in, out, S, rows, blk);
@@ -329,9 +381,11 @@ hipError_t launch_rank(int dtype, const View &in, const View &out, long S, long 
     return dtype ? launch_rank_t<double>(in, out, S, rows, blk, st) : launch_rank_t<float>(in, out, S, rows, blk, st);
 }
 
-hipError_t launch_scale(int dtype, const View &in, const View &out, long S, long rows, int blk, hipStream_t st) {
+hipError_t launch_scale(int dtype, const View &in, const View &out, long S, long rows, int blk, hipStream_t st,
+                        void *scratch) {
     if (rows <= 0) return hipSuccess;
-    return dtype ? launch_scale_t<double>(in, out, S, rows, blk, st) : launch_scale_t<float>(in, out, S, rows, blk, st);
+    return dtype ? launch_scale_t<double>(in, out, S, rows, blk, st, scratch)
+                 : launch_scale_t<float>(in, out, S, rows, blk, st, scratch);
 }
 
 } // namespace kun_cs

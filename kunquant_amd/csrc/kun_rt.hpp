@@ -58,6 +58,7 @@ struct BatchLayout {
         long zscr[KUN_MAX_COMPS] = {};
     };
     std::vector<std::vector<Kern>> kern;
+    std::vector<size_t> cs_scratch;          // per Scale stage: arena offset of its row sums
     std::vector<long> buf_off;                // per module buffer: arena offset of a temporary, else -1
     std::vector<std::vector<size_t>> mat_off; // per stage, per materialised value: arena offset
     size_t slot_bytes = 0;

@@ -380,9 +380,14 @@ static void plan_slots(const KunModuleDesc *m, size_t S, size_t L, size_t base, 
 BatchLayout batch_layout(const KunModuleDesc *m, size_t S, size_t L, const ChunkPolicy &pol, int lanes) {
     BatchLayout lay;
     lay.kern.resize(m->nstage);
+    lay.cs_scratch.assign(m->nstage, 0);
     size_t off = 0;
     for (int i = 0; i < m->nstage; i++) {
         const KunStageDesc &sd = m->stages[i];
+        if (sd.kind == KUN_STAGE_SCALE) {
+            lay.cs_scratch[i] = off;
+            off += align256(std::max<size_t>(1, L) * elem_size(m));
+        }
         if (sd.kind != KUN_STAGE_TS) continue;
         if (sd.nin + sd.nout + sd.nmat > 256) throw Error("stage with too many buffers");
         for (int j = 0; j < sd.nkern; j++) {
@@ -442,7 +447,8 @@ static void enqueue_node(const KunModuleDesc *m, int si, int kj, const std::vect
                         " stocks is not supported");
         hipError_t e = sd.kind == KUN_STAGE_RANK
                            ? kun_cs::launch_rank(m->dtype, vi, vo, (long)S, (long)L, m->blocking, st)
-                           : kun_cs::launch_scale(m->dtype, vi, vo, (long)S, (long)L, m->blocking, st);
+                           : kun_cs::launch_scale(m->dtype, vi, vo, (long)S, (long)L, m->blocking, st,
+                                                  arena + lay.cs_scratch[si]);
         if (e != hipSuccess) throw Error(std::string("cross-sectional kernel: ") + hipGetErrorString(e));
     }
     if (prof) {

@@ -29,6 +29,7 @@ from ..Driver import BUF_INPUT, BUF_TEMP, ModuleIR, STAGE_TS
 from ..Op import ConstantOp, ForeachBackWindow, Input, OpBase, Output, ReductionOp
 from .plan import PAR, SEQ, Component, StagePlan, plan_stage
 from .program import literal, program_text
+from .wire import wire_table
 
 TS_BLOCK = 64
 MAX_STAGE_BUFS = 256
@@ -465,6 +466,13 @@ class StageEmitter(_Lowering):
             if wr:
                 k = self.idx[wr[0]]
                 maxoff[k] = max(maxoff.get(k, 0), wr[1])
+        # a windowed temporary keeps its whole window even where no op of
+        # the stage reads that far: the rows are part of the serialised
+        # state (the reference stores them in a module buffer of that window)
+        for op in self.ops:
+            if op.__class__.__name__ == "WindowedTempOutput" and op.get_parent() is None:
+                k = self.idx[op]
+                maxoff[k] = max(maxoff.get(k, 0), op.attrs["window"] - 1)
         for k in sorted(maxoff):
             slots = _pow2(maxoff[k] + 1)
             self.ring[k] = (slots, self.n_t)
@@ -1216,12 +1224,14 @@ def emit_module_parts(module: ModuleIR, nparts: int, lds_budget: int = None) -> 
     split = nparts > 1 and not module.stream_mode
     kernel_srcs: List[Tuple[str, str]] = []  # (kernel name, source) of batch kernels
     meta = []
+    stream_ems = {}
     for st in module.stages:
         kname = f"kun_{mid}_s{st.idx}"
         if st.kind != STAGE_TS:
             meta.append((st, [], None, None, 0, 0))
         elif module.stream_mode:
             em = StageEmitter(module, st, kname)
+            stream_ems[st.idx] = em
             parts.append(em.stream_kernels())
             parts.append("")
             meta.append((st, [], f"{kname}_tick", f"{kname}_init", em.state_bytes, 0))
@@ -1309,6 +1319,26 @@ def emit_module_parts(module: ModuleIR, nparts: int, lds_budget: int = None) -> 
             parts.append(f"  {{{kind}, {kern}, kun_{mid}_ph{p}_stages, {len(sts)}}},")
         parts.append("};")
         pool = f"{len(phases)}, kun_{mid}_phases, (const void *)&kun_{mid}_pinit"
+    wire = "0, nullptr, 0, nullptr"
+    if module.stream_mode:
+        wbufs, wstates = wire_table(module, stream_ems)
+        for j, wb in enumerate(wbufs):
+            if wb.rings:
+                parts.append(f"static const KunRingRef kun_{mid}_w{j}[] = {{"
+                             + ", ".join(f"{{{r.stage}, {r.tslot}, {r.slots}}}" for r in wb.rings) + "};")
+        parts.append(f"static const KunWireBuf kun_{mid}_wbufs[] = {{")
+        for j, wb in enumerate(wbufs):
+            rings = f"kun_{mid}_w{j}" if wb.rings else "nullptr"
+            parts.append(f'  {{"{wb.name}", {wb.window}, {wb.buf}, {rings}, {len(wb.rings)}}},')
+        parts.append("};")
+        wst = f"kun_{mid}_wstates" if wstates else "nullptr"
+        if wstates:
+            parts.append(f"static const KunWireState kun_{mid}_wstates[] = {{")
+            for ws in wstates:
+                parts.append(f"  {{{ws.kind}, {ws.stage}, {ws.tslot}, {ws.islot}, {ws.window}, "
+                             f"{stream_ems[ws.stage].n_t}}},")
+            parts.append("};")
+        wire = f"{len(wbufs)}, kun_{mid}_wbufs, {len(wstates)}, {wst}"
     if module.stream_mode:
         parts.append(f"static const void *kun_{mid}_inits[] = {{"
                      + ", ".join(f"(const void *)&{ini}" if ini else "nullptr" for _, _, _, ini, _, _ in meta) + "};")
@@ -1327,7 +1357,7 @@ def emit_module_parts(module: ModuleIR, nparts: int, lds_budget: int = None) -> 
     parts.append(f"  KUN_MODULE_ABI_VERSION, \"{module.name}\", {0 if module.dtype == 'float' else 1},"
                  f" {lay[module.input_layout]}, {lay[module.output_layout]}, {module.blocking_len},"
                  f" {1 if module.aligned else 0}, {len(module.buffers)}, kun_{mid}_bufs, {len(module.stages)},"
-                 f" kun_{mid}_stages, kun_{mid}_program, {inits}, {pool}}};")
+                 f" kun_{mid}_stages, kun_{mid}_program, {inits}, {pool}, {wire}}};")
     if not split:
         return ["\n".join(parts) + "\n"]
     n = min(nparts - 1, len(kernel_srcs)) or 1

@@ -8,7 +8,7 @@
 #ifndef KUN_MODULE_ABI_H
 #define KUN_MODULE_ABI_H
 
-#define KUN_MODULE_ABI_VERSION 0x4B554E07u
+#define KUN_MODULE_ABI_VERSION 0x4B554E08u
 
 enum { KUN_BUF_INPUT = 0, KUN_BUF_OUTPUT = 1, KUN_BUF_TEMP = 2 };
 enum { KUN_STAGE_TS = 0, KUN_STAGE_RANK = 1, KUN_STAGE_SCALE = 2 };
@@ -81,6 +81,35 @@ typedef struct {
     int ndeps;
 } KunStageDesc;
 
+/* Stream-state wire table (STREAM modules): the parts of the reference's
+ * serialised stream state (Runtime.cpp:534-546) in its order, and where
+ * this backend keeps each of them. */
+typedef struct {
+    int stage;   /* time-series stage owning the ring */
+    int tslot;   /* first T slot of the ring in the stage's state */
+    int slots;   /* ring length, a power of two; tick t at slot t & (slots - 1) */
+} KunRingRef;
+
+typedef struct {
+    const char *name;        /* buffer name in the reference's buffer table */
+    int window;              /* rows the reference keeps */
+    int buf;                 /* module buffer holding the newest row; -1: a stage's windowed temporary */
+    const KunRingRef *rings; /* stage rings of the same values (newest row included) */
+    int nrings;
+} KunWireBuf;
+
+enum { KUN_WS_FASTSUM = 0, KUN_WS_EMA = 1, KUN_WS_LINREG = 2, KUN_WS_ACCUM = 3, KUN_WS_SKIPLIST = 4 };
+
+/* One state buffer of the reference (one object per blocking group). */
+typedef struct {
+    int kind;    /* KUN_WS_* */
+    int stage;
+    int tslot;   /* first T slot of the op's state */
+    int islot;   /* first int32 slot of the op's state */
+    int window;  /* window of the op (0 if none) */
+    int tslots;  /* T slots of the stage's state; its int32 slots follow them */
+} KunWireState;
+
 typedef struct {
     unsigned abi;
     const char *name;
@@ -100,6 +129,10 @@ typedef struct {
     int nphase;
     const KunPhaseDesc *phases;
     const void *pool_init;
+    int nwire_buf;
+    const KunWireBuf *wire_bufs;
+    int nwire_state;
+    const KunWireState *wire_states;
 } KunModuleDesc;
 
 #endif

@@ -100,14 +100,27 @@ def run_batch(so, name, inputs, outputs, S, T, start, L, dtype):
 
 
 # The reference C API streams float data only; this harness drives the C++
-# StreamContext (RunGraph.hpp:31-53) for both element types.
+# StreamContext (RunGraph.hpp:31-53) for both element types, including its
+# state serialisation (Runtime.cpp:534-546) and resume (Runtime.cpp:424-467).
 _HARNESS = r"""
 #include <Kun/RunGraph.hpp>
 #include <Kun/Module.hpp>
+#include <Kun/IO.hpp>
+#include <string.h>
 using namespace kun;
 extern "C" {
 void *h_create(const void *mod, size_t S) {
     return new StreamContext(createSingleThreadExecutor(), (const Module *)mod, S);
+}
+void *h_create_from(const void *mod, size_t S, const char *blob, size_t n) {
+    MemoryInputStream in(blob, n);
+    return new StreamContext(createSingleThreadExecutor(), (const Module *)mod, S, &in);
+}
+size_t h_serialize(void *c, char *out, size_t cap) {
+    MemoryOutputStream s;
+    if (!((StreamContext *)c)->serializeStates(&s)) return (size_t)-1;
+    if (out && s.getSize() <= cap) memcpy(out, s.getData(), s.getSize());
+    return s.getSize();
 }
 size_t h_query(void *c, const char *n) { return ((StreamContext *)c)->queryBufferHandle(n); }
 void h_push_f(void *c, size_t h, const float *d) { ((StreamContext *)c)->pushData(h, d); }
@@ -133,33 +146,55 @@ def harness():
     vp, sz = ctypes.c_void_p, ctypes.c_size_t
     for n, res, args in [("h_create", vp, [vp, sz]), ("h_query", sz, [vp, ctypes.c_char_p]),
                          ("h_push_f", None, [vp, sz, vp]), ("h_push_d", None, [vp, sz, vp]), ("h_run", None, [vp]),
-                         ("h_get_f", vp, [vp, sz]), ("h_get_d", vp, [vp, sz]), ("h_free", None, [vp])]:
+                         ("h_get_f", vp, [vp, sz]), ("h_get_d", vp, [vp, sz]), ("h_free", None, [vp]),
+                         ("h_create_from", vp, [vp, sz, ctypes.c_char_p, sz]),
+                         ("h_serialize", sz, [vp, ctypes.c_char_p, sz])]:
         fn = getattr(H, n)
         fn.restype = res
         fn.argtypes = args
     return H
 
 
-def run_stream(so, name, rows, outputs, S, dtype):
+def _serialize(H, st):
+    n = H.h_serialize(st, None, 0)
+    buf = ctypes.create_string_buffer(n)
+    assert H.h_serialize(st, buf, n) == n
+    return np.frombuffer(buf.raw, dtype=np.uint8).copy()
+
+
+def run_stream(so, name, rows, outputs, S, dtype, snapshots=(), resume=None):
+    """rows: {input name: [T, S]}. Returns ({output: [T, S]}, {tick: state
+    blob after that many ticks}); with resume=(blob, t0) the context starts
+    from the blob and is fed rows t0.. (outputs then hold those ticks)."""
     C = capi()
     H = harness()
     lib = C.kunLoadLibrary(so.encode())
     mod = C.kunGetModuleFromLibrary(lib, name.encode())
-    st = H.h_create(mod, S)
-    ha = H.h_query(st, b"a")
+    t0 = 0
+    if resume is not None:
+        blob, t0 = resume
+        st = H.h_create_from(mod, S, blob.tobytes(), blob.size)
+    else:
+        st = H.h_create(mod, S)
+    hin = {k: H.h_query(st, k.encode()) for k in rows}
     hs = {k: H.h_query(st, k.encode()) for k in outputs}
     dbl = dtype == np.float64
     ct = ctypes.c_double if dbl else ctypes.c_float
-    res = {k: np.empty((len(rows), S), dtype=dtype) for k in outputs}
-    for i, r in enumerate(rows):
-        r = np.ascontiguousarray(r, dtype=dtype)
-        (H.h_push_d if dbl else H.h_push_f)(st, ha, r.ctypes.data)
+    T = len(next(iter(rows.values())))
+    res = {k: np.empty((T - t0, S), dtype=dtype) for k in outputs}
+    blobs = {}
+    for i in range(t0, T):
+        for k, v in rows.items():
+            r = np.ascontiguousarray(v[i], dtype=dtype)
+            (H.h_push_d if dbl else H.h_push_f)(st, hin[k], r.ctypes.data)
         H.h_run(st)
         for k in outputs:
             p = (H.h_get_d if dbl else H.h_get_f)(st, hs[k])
-            res[k][i] = np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ct)), shape=(S,))
+            res[k][i - t0] = np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ct)), shape=(S,))
+        if i + 1 in snapshots:
+            blobs[i + 1] = _serialize(H, st)
     H.h_free(st)
-    return res
+    return res, blobs
 
 
 def output_names(f):
@@ -202,7 +237,9 @@ def main():
     if not os.path.exists(os.path.join(REF_OUT, "libKunRuntime.so")):
         subprocess.check_call(["bash", os.path.join(ROOT, "oracle", "build_ref.sh")])
     K = ref_namespace()
-    for name, factory, cfg, dtype, make_inputs, S, T, start, L in CASES:
+    # --stream-only: rewrite the stream fixtures alone
+    batch_cases = [] if "--stream-only" in sys.argv else CASES
+    for name, factory, cfg, dtype, make_inputs, S, T, start, L in batch_cases:
         f = factory(K)
         outs = output_names(f)
         so = ref_compile(K, f, name, dict(cfg))
@@ -220,10 +257,29 @@ def main():
         S = 16
         rows = rng.random((60, S)).astype(dtype)
         rows[13, 2] = np.nan
-        res = run_stream(so, name, rows, outs, S, dtype)
+        res, blobs = run_stream(so, name, {"a": rows}, outs, S, dtype, snapshots=(37, 60))
+        # resuming from the blob reproduces the uninterrupted run
+        tail, _ = run_stream(so, name, {"a": rows}, outs, S, dtype, resume=(blobs[37], 37))
+        for k in outs:
+            assert np.array_equal(tail[k], res[k][37:], equal_nan=True), k
         np.savez_compressed(os.path.join(HERE, f"ref_{name}.npz"), in_a=rows,
-                            **{f"out_{k}": v for k, v in res.items()}, meta=np.array([S, len(rows)]))
-        print("wrote", name)
+                            **{f"out_{k}": v for k, v in res.items()}, meta=np.array([S, len(rows)]),
+                            state37=blobs[37], state60=blobs[60])
+        print("wrote", name, "state blobs", blobs[37].size, blobs[60].size)
+    # stream-state blobs of the 100-factor serving graph (windowed stage
+    # temporaries, rolling sums, regressions and skip lists across stages)
+    name, S, T, K1 = "wire_serve_f32", 16, 90, 71
+    f = wgraphs.random_graph(100, 5000, K=K)
+    outs = output_names(f)
+    so = ref_compile(K, f, name, dict(input_layout="STREAM", output_layout="STREAM", options={"opt_reduce": False}))
+    d = wdata.ohlcv(S, T, seed=12, dtype=np.float32, nan_frac=0.05)
+    ins = [op.attrs["name"] for op in f.ops if op.__class__.__name__ == "Input"]
+    rows = {k: d[k] for k in ins}
+    res, blobs = run_stream(so, name, rows, outs, S, np.float32, snapshots=(K1, T))
+    np.savez_compressed(os.path.join(HERE, f"ref_{name}.npz"), **{f"in_{k}": v for k, v in rows.items()},
+                        **{f"out_{k}": v[-3:] for k, v in res.items()}, meta=np.array([S, T, K1]),
+                        state_k1=blobs[K1], state_end=blobs[T])
+    print("wrote", name, "state blobs", blobs[K1].size, blobs[T].size)
 
 
 if __name__ == "__main__":

@@ -62,7 +62,7 @@ def post_optimize(impl: List[Function], options: dict) -> Dict[str, int]:
 
 
 BUF_INPUT, BUF_OUTPUT, BUF_TEMP = "INPUT", "OUTPUT", "TEMP"
-STAGE_TS, STAGE_RANK, STAGE_SCALE = "ts", "rank", "scale"
+STAGE_TS, STAGE_RANK, STAGE_SCALE, STAGE_GENERIC = "ts", "rank", "scale", "generic"
 
 
 @dataclass
@@ -237,8 +237,8 @@ def compileit(f: Function, module_name: str, partition_factor=3, dtype="float", 
                 if isinstance(op, CrossSectionalOp):
                     raise RuntimeError(f"Cross sectional op must be alone in its stage: {op}")
         elif isinstance(cs, GenericCrossSectionalOp):
-            raise RuntimeError(f"{cs.__class__.__name__}: GenericCrossSectionalOp carries a CPU C++ body "
-                               f"and cannot run on the MI355X backend")
+            # its body is compiled as device code (one thread per row)
+            kind = STAGE_GENERIC
         else:
             kind = STAGE_RANK if cs.__class__.__name__ == "Rank" else STAGE_SCALE
         st = StageIR(len(stages), func.name, kind, func, ins, outs)

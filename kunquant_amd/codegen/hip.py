@@ -25,7 +25,7 @@ import os
 import re
 from typing import Dict, List, Optional, Tuple
 
-from ..Driver import BUF_INPUT, BUF_TEMP, ModuleIR, STAGE_TS
+from ..Driver import BUF_INPUT, BUF_TEMP, ModuleIR, STAGE_GENERIC, STAGE_TS
 from ..Op import ConstantOp, ForeachBackWindow, Input, OpBase, Output, ReductionOp
 from .plan import PAR, SEQ, Component, StagePlan, plan_stage
 from .program import literal, program_text
@@ -1212,6 +1212,40 @@ def emit_module(module: ModuleIR, lds_budget: int = None) -> str:
     return emit_module_parts(module, 1, lds_budget)[0]
 
 
+CS_MAX_BUFS = 16  # KUN_MAX_CS_BUFS
+
+
+def generic_cs_kernel(module: ModuleIR, stage, kname: str) -> str:
+    """Row kernel of a generic cross-sectional stage: the op's head and body
+    (C++ statements, MiscOp.GenericCrossSectionalOp) compiled as device
+    code, one thread per row. As in the reference (CodegenCpp.py:111-137),
+    input_<n> is the stage's n-th input buffer and output_<n> its n-th
+    output buffer."""
+    op = next(o for o in stage.func.ops if not isinstance(o, (Input, Output)))
+    nin, nout = len(stage.in_bufs), len(stage.out_bufs)
+    if nin + nout > CS_MAX_BUFS:
+        raise RuntimeError(f"{op.__class__.__name__}: more than {CS_MAX_BUFS} buffers in one cross-sectional stage")
+    T = "float" if module.dtype == "float" else "double"
+    L = [f'extern "C" __global__ __launch_bounds__({TS_BLOCK}) void {kname}_cs(kun::CsArgs A) {{',
+         f"  const long __row = (long)blockIdx.x * {TS_BLOCK} + threadIdx.x;",
+         "  if (__row >= A.L) return;",
+         f"  using T = {T};",
+         "  const size_t num_stocks = (size_t)A.S;",
+         "  (void)num_stocks;"]
+    names = [f"input_{n}" for n in range(nin)] + [f"output_{n}" for n in range(nout)]
+    for j, nm in enumerate(names):
+        L.append(f"  const kun::CsRow<T> {nm}{{(T *)A.bufs[{j}], A.lay[{j}], A.S, A.rows[{j}], "
+                 f"A.row0[{j}] + __row, A.blk}};")
+    L.append("  {")
+    L.append(op.generate_head())
+    L.append("  {")
+    L.append(op.generate_body())
+    L.append("  }")
+    L.append("  }")
+    L.append("}")
+    return "\n".join(L)
+
+
 def emit_module_parts(module: ModuleIR, nparts: int, lds_budget: int = None) -> List[str]:
     """The module as up to `nparts` translation units compiled in parallel:
     batch stage kernels spread over the first units by source size, the
@@ -1223,11 +1257,21 @@ def emit_module_parts(module: ModuleIR, nparts: int, lds_budget: int = None) -> 
     parts = list(header)
     split = nparts > 1 and not module.stream_mode
     kernel_srcs: List[Tuple[str, str]] = []  # (kernel name, source) of batch kernels
+    cs_args = set()                           # those of them taking kun::CsArgs
     meta = []
     stream_ems = {}
     for st in module.stages:
         kname = f"kun_{mid}_s{st.idx}"
-        if st.kind != STAGE_TS:
+        if st.kind == STAGE_GENERIC:
+            src = generic_cs_kernel(module, st, kname)
+            if split:
+                kernel_srcs.append((f"{kname}_cs", src))
+                cs_args.add(f"{kname}_cs")
+            else:
+                parts.append(src)
+                parts.append("")
+            meta.append((st, [], f"{kname}_cs", None, 0, 0))
+        elif st.kind != STAGE_TS:
             meta.append((st, [], None, None, 0, 0))
         elif module.stream_mode:
             em = StageEmitter(module, st, kname)
@@ -1269,7 +1313,7 @@ def emit_module_parts(module: ModuleIR, nparts: int, lds_budget: int = None) -> 
     for b in module.buffers:
         parts.append(f'  {{"{b.name}", {kinds[b.kind]}, {b.unreliable}, {b.window}}},')
     parts.append("};")
-    stage_kind = {"ts": 0, "rank": 1, "scale": 2}
+    stage_kind = {"ts": 0, "rank": 1, "scale": 2, "generic": 3}
     for st, kerns, *_ in meta:
         for tag, arr in (("in", [b.idx for b in st.in_bufs]), ("out", [b.idx for b in st.out_bufs]),
                          ("dep", st.deps)):
@@ -1369,6 +1413,7 @@ def emit_module_parts(module: ModuleIR, nparts: int, lds_budget: int = None) -> 
         sizes[j] += len(src)
     # host-only unit: the descriptor points at the kernels' host stubs, which
     # the other units define; its device pass has nothing to compile
-    decls = [f'extern "C" __global__ void {name}(kun::StageArgs A);' for name, _ in kernel_srcs]
+    decls = [f'extern "C" __global__ void {name}(kun::{"CsArgs" if name in cs_args else "StageArgs"} A);'
+             for name, _ in kernel_srcs]
     desc = header + ["#if !defined(__HIP_DEVICE_COMPILE__)"] + decls + [""] + parts[len(header):] + ["#endif"]
     return ["\n".join(u) + "\n" for u in units] + ["\n".join(desc) + "\n"]

@@ -8,12 +8,22 @@ decimal string the reference emits for its C++ literals
 decimal->float exactly as the reference compiler does."""
 from typing import Dict, List
 
-from ..Driver import BUF_INPUT, BUF_OUTPUT, BUF_TEMP, ModuleIR, STAGE_RANK, STAGE_SCALE, STAGE_TS
+from ..Driver import BUF_INPUT, BUF_OUTPUT, BUF_TEMP, ModuleIR, STAGE_GENERIC, STAGE_RANK, STAGE_SCALE, STAGE_TS
 from ..Op import (ConstantOp, ForeachBackWindow, Input, OpBase, Output, ReductionOp,
                   WindowedTempOutput)
 
 _KIND_CODE = {BUF_INPUT: 0, BUF_OUTPUT: 1, BUF_TEMP: 2}
 _STAGE_CODE = {STAGE_TS: 0, STAGE_RANK: 1, STAGE_SCALE: 2}
+# generic cross-sectional stages: the ones the oracle knows by name, and 9
+# for a user body the oracle cannot run
+_GENERIC_CODE = {"DiffWithWeightedSum": 3}
+
+
+def stage_code(stage) -> int:
+    if stage.kind != STAGE_GENERIC:
+        return _STAGE_CODE[stage.kind]
+    op = next(o for o in stage.func.ops if not isinstance(o, (Input, Output)))
+    return _GENERIC_CODE.get(op.__class__.__name__, 9)
 _LAYOUT_CODE = {"STs": 0, "TS": 1, "STREAM": 2}
 
 
@@ -82,7 +92,7 @@ def program_text(module: ModuleIR) -> str:
     out.append(f"stages {len(module.stages)}")
     for st in module.stages:
         ops = stage_ops(module, st) if st.kind == STAGE_TS else []
-        head = ["S", str(st.idx), str(_STAGE_CODE[st.kind]), str(len(ops)),
+        head = ["S", str(st.idx), str(stage_code(st)), str(len(ops)),
                 str(len(st.in_bufs)), *[str(b.idx) for b in st.in_bufs],
                 str(len(st.out_bufs)), *[str(b.idx) for b in st.out_bufs],
                 str(len(st.deps)), *[str(d) for d in st.deps]]

@@ -456,6 +456,34 @@ struct StreamArgs {
     long S;
     long tick;
 };
+// Generic cross-sectional stage (a GenericCrossSectionalOp's body compiled
+// as device code): one thread per row. Buffer n (stage inputs, then
+// outputs) is addressed in layout lay[n] -- 0 STs (rows[n] rows, blocking
+// blk), 1 TS, 2 a single row -- and computed row r is its row row0[n] + r.
+#define KUN_MAX_CS_BUFS 16
+struct CsArgs {
+    void *bufs[KUN_MAX_CS_BUFS];
+    int lay[KUN_MAX_CS_BUFS];
+    long rows[KUN_MAX_CS_BUFS];
+    long row0[KUN_MAX_CS_BUFS];
+    long S;
+    long L;
+    int blk;
+};
+// input_<n> / output_<n> of a body: element i = stock i of one row
+template <typename T> struct CsRow {
+    T *p;
+    int lay;
+    long S, rows, row;
+    int blk;
+    __device__ __forceinline__ T &operator[](size_t i) const {
+        const long s = (long)i;
+        const size_t at = lay == 1   ? (size_t)row * S + s
+                          : lay == 2 ? (size_t)s
+                                     : ((size_t)(s / blk) * rows + row) * blk + s % blk;
+        return p[at];
+    }
+};
 // Session pools: one entry per workgroup of a pool launch -- the lanes
 // [s0, s1) of one session's stock range (at most one wavefront), where the
 // session's rows start in the packed input rows of the tick (p0), and the

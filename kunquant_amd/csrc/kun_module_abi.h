@@ -11,7 +11,7 @@
 #define KUN_MODULE_ABI_VERSION 0x4B554E08u
 
 enum { KUN_BUF_INPUT = 0, KUN_BUF_OUTPUT = 1, KUN_BUF_TEMP = 2 };
-enum { KUN_STAGE_TS = 0, KUN_STAGE_RANK = 1, KUN_STAGE_SCALE = 2 };
+enum { KUN_STAGE_TS = 0, KUN_STAGE_RANK = 1, KUN_STAGE_SCALE = 2, KUN_STAGE_GENERIC = 3 };
 enum { KUN_LAYOUT_STS = 0, KUN_LAYOUT_TS = 1, KUN_LAYOUT_STREAM = 2 };
 enum { KUN_DTYPE_FLOAT = 0, KUN_DTYPE_DOUBLE = 1 };
 enum { KUN_KERNEL_PAR = 0, KUN_KERNEL_SEQ = 1 };
@@ -70,7 +70,8 @@ typedef struct {
     const KunKernelDesc *kerns; /* batch kernels of a TS stage, run in order */
     int nkern;
     int nmat;             /* [rows][stocks] values handed between the stage's kernels */
-    const void *stream_kernel; /* one-tick kernel host stub (STREAM modules) */
+    const void *stream_kernel; /* one-tick kernel host stub (STREAM modules); KUN_STAGE_GENERIC:
+                                  its row kernel (kun::CsArgs), batch and stream */
     int block;            /* threads per workgroup */
     int state_bytes;      /* per-stock HBM bytes of carried state (STREAM modules) */
     const int *in;

@@ -438,6 +438,26 @@ static void enqueue_node(const KunModuleDesc *m, int si, int kj, const std::vect
         void *args[] = {&a};
         if (kl.blocks > 0x7fffffffUL) throw Error("batch too large for one launch");
         HIPCHECK(hipLaunchKernel(kd.kernel, dim3((unsigned)kl.blocks), dim3(sd.block), args, 0, st));
+    } else if (sd.kind == KUN_STAGE_GENERIC) {
+        kun::CsArgs a;
+        memset(&a, 0, sizeof a);
+        int n = 0;
+        auto add = [&](int b) {
+            kun_cs::View v = cs_view(m, b, dev[b], T, start, L);
+            a.bufs[n] = v.ptr;
+            a.lay[n] = v.layout;
+            a.rows[n] = v.rows;
+            a.row0[n] = v.row0;
+            n++;
+        };
+        if (sd.nin + sd.nout > KUN_MAX_CS_BUFS) throw Error("too many buffers in a cross-sectional stage");
+        for (int j = 0; j < sd.nin; j++) add(sd.in[j]);
+        for (int j = 0; j < sd.nout; j++) add(sd.out[j]);
+        a.S = (long)S;
+        a.L = (long)L;
+        a.blk = m->blocking;
+        void *args[] = {&a};
+        HIPCHECK(hipLaunchKernel(sd.stream_kernel, dim3((unsigned)((L + 63) / 64)), dim3(64), args, 0, st));
     } else {
         int ib = sd.in[0], ob = sd.out[0];
         kun_cs::View vi = cs_view(m, ib, dev[ib], T, start, L);

@@ -383,6 +383,21 @@ void stream_run(StreamCtx *c) {
         const KunStageDesc &sd = m->stages[si];
         if (sd.kind == KUN_STAGE_TS) {
             launch_stage_kernel(sd.stream_kernel, c, si, (long)c->tick);
+        } else if (sd.kind == KUN_STAGE_GENERIC) {
+            kun::CsArgs a;
+            memset(&a, 0, sizeof a);
+            int n = 0;
+            for (int j = 0; j < sd.nin; j++) a.bufs[n++] = c->rows[sd.in[j]];
+            for (int j = 0; j < sd.nout; j++) a.bufs[n++] = c->rows[sd.out[j]];
+            for (int j = 0; j < n; j++) {
+                a.lay[j] = 2;
+                a.rows[j] = 1;
+            }
+            a.S = (long)c->S;
+            a.L = 1;
+            a.blk = m->blocking;
+            void *args[] = {&a};
+            HIPCHECK(hipLaunchKernel(sd.stream_kernel, dim3(1), dim3(64), args, 0, c->ex->stream));
         } else {
             kun_cs::View vi{c->rows[sd.in[0]], 2, 1, 0}, vo{c->rows[sd.out[0]], 2, 1, 0};
             hipError_t e = sd.kind == KUN_STAGE_RANK

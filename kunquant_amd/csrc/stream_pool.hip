@@ -230,6 +230,9 @@ StreamPool *pool_create(std::shared_ptr<Executor> ex, const KunModuleDesc *m, si
     for (int b = 0; b < m->nbuf; b++)
         if (m->bufs[b].kind == KUN_BUF_INPUT) p->inputs.push_back(b);
     if ((int)p->inputs.size() > POOL_MAX_INPUTS) throw Error("session pools take at most 16 inputs");
+    for (int si = 0; si < m->nstage; si++)
+        if (m->stages[si].kind == KUN_STAGE_GENERIC)
+            throw Error("session pools do not run generic cross-sectional stages; use a stream context");
     size_t max_segs = 0;
     for (int ph = 0; ph < m->nphase; ph++) {
         const KunPhaseDesc &pd = m->phases[ph];

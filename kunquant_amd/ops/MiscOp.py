@@ -165,8 +165,12 @@ class SkipListArgMin(SkipListConsumerOp, OpBase):
 
 
 class GenericCrossSectionalOp(CrossSectionalOp):
-    """Cross-sectional op with a user-supplied CPU body. Not executable on the
-    device backend: compilation rejects it with an explicit error."""
+    """Cross-sectional op with a user-supplied body of C++ statements over one
+    row (MiscOp.py:201-224 of the reference). The body is compiled as HIP
+    device code and runs one device thread per row. Names in scope:
+    `T` (float or double), `num_stocks`, `input_<n>[i]` (input n at stock
+    i of the row), `output_<n>[i]`; `generate_head()` runs first, in the
+    same scope. Device code only: no host library calls."""
 
     def generate_body(self) -> str:
         raise NotImplementedError("GenericCrossSectionalOp must be specialized")
@@ -176,8 +180,27 @@ class GenericCrossSectionalOp(CrossSectionalOp):
 
 
 class DiffWithWeightedSum(GenericCrossSectionalOp):
+    """out[i] = v[i] - sum_j v[j] * w[j] over the row (MiscOp.py:227-249 of
+    the reference): the sum runs over the stocks in order, products and
+    additions rounded one at a time."""
+
     def __init__(self, v: OpBase, w: OpBase) -> None:
         super().__init__([v, w], None)
+
+    def generate_head(self) -> str:
+        return ""
+
+    def generate_body(self) -> str:
+        return """
+        T acc = 0;
+        for (size_t i = 0; i < num_stocks; ++i) {
+            const T prod = input_0[i] * input_1[i];
+            acc = acc + prod;
+        }
+        for (size_t i = 0; i < num_stocks; ++i) {
+            output_0[i] = input_0[i] - acc;
+        }
+        """
 
 
 class GenericPartition(OpBase, SinkOpTrait):

@@ -6,7 +6,7 @@ import os
 from typing import Callable, Dict, Tuple
 
 from .. import PACKAGE_DIR
-from .graphs import (aligned_graph, cross_section_graph, ops_graph, precision_graph, random_graph, simple_graph,
+from .graphs import (aligned_graph, cross_section_graph, generic_cs_graph, ops_graph, precision_graph, random_graph, simple_graph,
                      stream_graph)
 
 MODULE_DIR = os.path.join(PACKAGE_DIR, "lib", "modules")
@@ -27,6 +27,11 @@ CATALOG: Dict[str, Tuple[Callable, dict]] = {
                                                   options={"no_fast_stat": True})),
     "aligned_f32": (aligned_graph, dict(input_layout="TS", output_layout="TS", allow_unaligned=False)),
     "cs_f32": (cross_section_graph, dict(input_layout="TS", output_layout="TS")),
+    # generic cross-sectional op (device-compiled body) in TS, STs and stream modules
+    "generic_f32": (generic_cs_graph, dict(input_layout="TS", output_layout="TS")),
+    "generic_f64_sts": (generic_cs_graph, dict(dtype="double", input_layout="STs", output_layout="STs",
+                                               options={"no_fast_stat": "no_warn"})),
+    "generic_stream_f32": (generic_cs_graph, dict(input_layout="STREAM", output_layout="STREAM")),
     # graphs of the batched-jobs workload (64 uneven jobs draw from this pool)
     "jobs_g0": (lambda: random_graph(20, 4000), dict(input_layout="TS", output_layout="TS")),
     "jobs_g1": (lambda: random_graph(30, 4001), dict(input_layout="TS", output_layout="TS")),

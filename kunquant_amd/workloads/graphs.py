@@ -29,7 +29,7 @@ _NAMES = ["Builder", "Input", "Output", "Rank", "Scale", "ConstantOp", "Function
           "WindowedMin", "WindowedMax", "ExpMovingAvg", "WindowedQuantile", "WindowedLinearRegressionSlope",
           "WindowedLinearRegressionRSqaure", "WindowedLinearRegressionResi", "WindowedSkew", "WindowedKurt",
           "WindowedMaxDrawdown", "Log", "Exp", "Sqrt", "Sign", "SetInfOrNanToValue", "Max", "Min", "Pow", "Clip",
-          "And", "Or", "Not", "LessEqual", "Equals", "WindowedVar"]
+          "And", "Or", "Not", "LessEqual", "Equals", "WindowedVar", "DiffWithWeightedSum"]
 
 
 def namespace(op_mod, ops_mod, stage_mod):
@@ -244,6 +244,21 @@ def aligned_graph(K=None):
     b = K.Builder()
     with b:
         K.Output(K.Rank(K.Input("a")) * 2, "ou2")
+    return K.Function(b.ops)
+
+
+def generic_cs_graph(K=None):
+    """A generic cross-sectional op (body compiled as device code) between
+    time-series stages: DiffWithWeightedSum(a - b, c) (the reference's
+    test_runtime.py:78-99 graph), its 5-row average and its rank."""
+    K = K or native()
+    b = K.Builder()
+    with b:
+        a, bb, c = K.Input("a"), K.Input("b"), K.Input("c")
+        d = K.DiffWithWeightedSum(a - bb, c)
+        K.Output(d, "out")
+        K.Output(K.WindowedAvg(d, 5), "avg5")
+        K.Output(K.Rank(d), "rank")
     return K.Function(b.ops)
 
 

@@ -135,6 +135,7 @@ void *kun_oracle_load(const char *text, char *err, int errlen) {
             s = &pg->stages[id];
             s->id = id;
             s->kind = (int)strtol(q, &q, 10);
+            if (s->kind < 0 || s->kind > 3) FAIL("stage kind the oracle cannot run (user cross-sectional body)");
             s->nops = (int)strtol(q, &q, 10);
             s->nin = (int)strtol(q, &q, 10);
             if (s->nin > 256) FAIL("too many inputs");
@@ -273,23 +274,42 @@ static void *ts_worker_f64(void *arg) {
     return NULL;
 }
 
-/* cross-sectional rows: gather row, compute, scatter */
+/* cross-sectional rows: gather row(s), compute, scatter. Stage kinds:
+ * 1 RankStocks, 2 ScaleStocks, 3 DiffWithWeightedSum (MiscOp.py:227-249:
+ * acc = sum over stocks in order of in0*in1, each product and addition
+ * rounded in T; out = in0 - acc). */
 #define CS_WORKER(SFX, REALT)                                                                  \
+    static void gather_##SFX(Job *j, int b, long t, REALT *row) {                              \
+        BufView_##SFX *v = (BufView_##SFX *)j->views;                                          \
+        const Buf *bi = &j->pg->bufs[b];                                                       \
+        long r = (bi->kind == 0 ? j->start : 0) + t - v[b].base;                               \
+        for (long s = 0; s < j->S; s++)                                                        \
+            row[s] = ((REALT *)v[b].ptr)[buf_index_##SFX(&v[b], r, s, j->S, j->pg->blocking)];   \
+    }                                                                                          \
     static void *cs_worker_##SFX(void *arg) {                                                  \
         Job *j = (Job *)arg;                                                                   \
         BufView_##SFX *v = (BufView_##SFX *)j->views;                                          \
-        int ib = j->sg->in[0], ob = j->sg->out[0];                                             \
-        const Buf *bi = &j->pg->bufs[ib];                                                      \
-        REALT *row = (REALT *)malloc(sizeof(REALT) * j->S * 3);                                \
-        REALT *res = row + j->S, *tmp = row + 2 * j->S;                                        \
+        int ob = j->sg->out[0];                                                                \
+        REALT *row = (REALT *)malloc(sizeof(REALT) * j->S * 4);                                \
+        REALT *res = row + j->S, *tmp = row + 2 * j->S, *row2 = row + 3 * j->S;                \
         for (long t = j->item0; t < j->item1; t++) {                                           \
-            long r = (bi->kind == 0 ? j->start : 0) + t - v[ib].base;                           \
-            for (long s = 0; s < j->S; s++)                                                    \
-                row[s] = ((REALT *)v[ib].ptr)[buf_index_##SFX(&v[ib], r, s, j->S, j->pg->blocking)]; \
+            gather_##SFX(j, j->sg->in[0], t, row);                                             \
             if (j->sg->kind == 1) rank_row_##SFX(row, res, j->S, tmp);                         \
-            else scale_row_##SFX(row, res, j->S);                                              \
-            for (long s = 0; s < j->S; s++)                                                    \
-                ((REALT *)v[ob].ptr)[buf_index_##SFX(&v[ob], t, s, j->S, j->pg->blocking)] = res[s]; \
+            else if (j->sg->kind == 2) scale_row_##SFX(row, res, j->S);                        \
+            else {                                                                             \
+                gather_##SFX(j, j->sg->in[1], t, row2);                                        \
+                REALT acc = 0;                                                                 \
+                for (long s = 0; s < j->S; s++) {                                              \
+                    REALT prod = row[s] * row2[s];                                             \
+                    acc = acc + prod;                                                          \
+                }                                                                              \
+                for (long s = 0; s < j->S; s++) res[s] = row[s] - acc;                         \
+            }                                                                                  \
+            for (int o = 0; o < j->sg->nout; o++) {                                            \
+                ob = j->sg->out[o];                                                            \
+                for (long s = 0; s < j->S; s++)                                                \
+                    ((REALT *)v[ob].ptr)[buf_index_##SFX(&v[ob], t, s, j->S, j->pg->blocking)] = res[s]; \
+            }                                                                                  \
         }                                                                                      \
         free(row);                                                                             \
         return NULL;                                                                           \

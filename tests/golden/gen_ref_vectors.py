@@ -211,8 +211,17 @@ def ops_inputs(S, T, dtype, seed):
     return {"a": a, "b": b}
 
 
+def generic_inputs(S, T, dtype, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    d = {k: rng.random((T, S)).astype(dtype) for k in ("a", "b", "c")}
+    d["a"][5, 7] = np.nan
+    return d
+
+
 CASES = [
     # name, graph factory, compile config, dtype, inputs(S, T), S, T, start, L
+    ("generic_f32", wgraphs.generic_cs_graph, dict(input_layout="TS", output_layout="TS"), np.float32,
+     lambda: generic_inputs(40, 60, np.float32, 21), 40, 60, 0, 60),
     ("simple_f32", wgraphs.simple_graph, dict(input_layout="TS", output_layout="TS"), np.float32,
      lambda: {k: v for k, v in wdata.ohlcv(64, 260, seed=1).items() if k != "amount"}, 64, 260, 0, 260),
     ("ops_f32", wgraphs.ops_graph, dict(input_layout="TS", output_layout="TS"), np.float32,
@@ -237,8 +246,9 @@ def main():
     if not os.path.exists(os.path.join(REF_OUT, "libKunRuntime.so")):
         subprocess.check_call(["bash", os.path.join(ROOT, "oracle", "build_ref.sh")])
     K = ref_namespace()
-    # --stream-only: rewrite the stream fixtures alone
-    batch_cases = [] if "--stream-only" in sys.argv else CASES
+    # --stream-only: rewrite the stream fixtures alone; --only=<case>: one batch case
+    only = [a.split("=", 1)[1] for a in sys.argv if a.startswith("--only=")]
+    batch_cases = [] if "--stream-only" in sys.argv else [c for c in CASES if not only or c[0] in only]
     for name, factory, cfg, dtype, make_inputs, S, T, start, L in batch_cases:
         f = factory(K)
         outs = output_names(f)
@@ -249,6 +259,8 @@ def main():
                             **{f"in_{k}": v for k, v in inputs.items()}, **{f"out_{k}": v for k, v in res.items()},
                             meta=np.array([S, T, start, L]))
         print("wrote", name, len(outs), "outputs")
+    if only:
+        return
     rng = np.random.Generator(np.random.PCG64(11))
     for name, cfg, dtype in STREAM_CASES:
         f = wgraphs.stream_graph(K)

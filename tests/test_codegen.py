@@ -1,4 +1,6 @@
 """HIP emitter and driver checks that need no GPU."""
+import os
+
 import pytest
 
 from kunquant_amd.Driver import KunCompilerConfig, compileit
@@ -42,14 +44,39 @@ def test_driver_option_errors():
         compileit(g(), "m", output_layout="STREAM", allow_unaligned=True)
 
 
-def test_generic_cross_sectional_rejected():
-    class Diff(DiffWithWeightedSum):
-        pass
+def test_generic_cross_sectional_body_compiles_to_device_code(tmp_path):
+    """A user's GenericCrossSectionalOp: its head and body become a device
+    kernel (one thread per row) that hipcc compiles for gfx950; the oracle,
+    which cannot run a user body, refuses the program."""
+    from kunquant_amd.build import compile_unit
+    from kunquant_amd.codegen.hip import emit_module_parts
+    from kunquant_amd.codegen.program import program_text
+    from kunquant_amd.ops import GenericCrossSectionalOp
+    from oracle.kun_oracle import OracleProgram
+
+    class Demean(GenericCrossSectionalOp):
+        def __init__(self, v):
+            super().__init__([v], None)
+
+        def generate_head(self):
+            return "T mean = 0;"
+
+        def generate_body(self):
+            return """
+            for (size_t i = 0; i < num_stocks; ++i) mean = mean + input_0[i];
+            mean = mean / (T)num_stocks;
+            for (size_t i = 0; i < num_stocks; ++i) output_0[i] = input_0[i] - mean;
+            """
     b = Builder()
     with b:
-        Output(Diff(Input("a"), Input("b")), "o")
-    with pytest.raises(RuntimeError, match="cannot run on the MI355X backend"):
-        compileit(Function(b.ops), "m", input_layout="TS", output_layout="TS")
+        Output(Demean(Input("a")), "o")
+    m = compileit(Function(b.ops), "m", input_layout="TS", output_layout="TS")
+    assert [s.kind for s in m.stages] == ["generic"]
+    src = emit_module_parts(m, 1)[0]
+    assert "void kun_m_s0_cs(kun::CsArgs A)" in src and "mean = mean / (T)num_stocks;" in src
+    assert os.path.exists(compile_unit(src, str(tmp_path), 0))
+    with pytest.raises(RuntimeError, match="cannot run"):
+        OracleProgram(program_text(m))
 
 
 def test_ema_init_rejected_in_stream():

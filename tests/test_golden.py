@@ -22,6 +22,8 @@ BATCH = {
     "rand_f32": (lambda: graphs.random_graph(100, 102), dict(input_layout="TS", output_layout="TS")),
     "rand_f64": (lambda: graphs.random_graph(40, 103), dict(dtype="double", input_layout="TS", output_layout="TS",
                                                             options={"no_fast_stat": "no_warn"})),
+    # generic cross-sectional op (DiffWithWeightedSum) between time-series stages
+    "generic_f32": (graphs.generic_cs_graph, dict(input_layout="TS", output_layout="TS")),
 }
 STREAM = {
     "stream_f64": dict(dtype="double", input_layout="STREAM", output_layout="STREAM"),

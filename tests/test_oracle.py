@@ -447,3 +447,19 @@ def test_oracle_aggregate_matches_pandas(dtype):
     sub = ko.aggregate(a, labels, start=30)
     g2 = pd.DataFrame(a[30:], index=stamps[30:]).groupby(stamps[30:].date)
     np.testing.assert_allclose(sub["sum"], g2.sum(), atol=1e-6, rtol=1e-4)
+
+
+def test_diff_with_weighted_sum():  # test_runtime.py:78-99 (atol=1e-6, rtol=1e-4)
+    from kunquant_amd.Driver import compileit
+    from kunquant_amd.codegen.program import program_text
+    from kunquant_amd.workloads.graphs import generic_cs_graph
+    from oracle.kun_oracle import OracleProgram
+    rng = np.random.default_rng(9)
+    ins = {k: rng.random((24, 16)).astype(np.float32) for k in ("a", "b", "c")}
+    ins["a"][3, 4] = np.nan
+    prog = OracleProgram(program_text(compileit(generic_cs_graph(), "g", input_layout="TS", output_layout="TS")))
+    got = prog.run(ins, 16, 24, 0, 24)
+    v1 = (ins["a"] - ins["b"]).astype(np.float64)
+    exp = v1 - np.sum(v1 * ins["c"], axis=1).reshape(-1, 1)
+    np.testing.assert_allclose(got["out"], exp, atol=1e-6, rtol=1e-4, equal_nan=True)
+    assert np.isnan(got["out"][3]).all() and not np.isnan(np.delete(got["out"], 3, axis=0)).any()

@@ -6,7 +6,7 @@ import os
 from typing import Callable, Dict, Tuple
 
 from .. import PACKAGE_DIR
-from .graphs import (aligned_graph, cross_section_graph, generic_cs_graph, ops_graph, precision_graph, random_graph, simple_graph,
+from .graphs import (aligned_graph, alpha158_graph, cross_section_graph, generic_cs_graph, ops_graph, precision_graph, random_graph, simple_graph,
                      stream_graph)
 
 MODULE_DIR = os.path.join(PACKAGE_DIR, "lib", "modules")
@@ -27,6 +27,9 @@ CATALOG: Dict[str, Tuple[Callable, dict]] = {
                                                   options={"no_fast_stat": True})),
     "aligned_f32": (aligned_graph, dict(input_layout="TS", output_layout="TS", allow_unaligned=False)),
     "cs_f32": (cross_section_graph, dict(input_layout="TS", output_layout="TS")),
+    # the Alpha158 factor library as the reference's test compiles it
+    "alpha158_f64": (alpha158_graph, dict(dtype="double", blocking_len=4, partition_factor=4, input_layout="TS",
+                                          output_layout="TS", options={"opt_reduce": True, "fast_log": True})),
     # generic cross-sectional op (device-compiled body) in TS, STs and stream modules
     "generic_f32": (generic_cs_graph, dict(input_layout="TS", output_layout="TS")),
     "generic_f64_sts": (generic_cs_graph, dict(dtype="double", input_layout="STs", output_layout="STs",

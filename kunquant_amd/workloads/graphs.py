@@ -247,6 +247,33 @@ def aligned_graph(K=None):
     return K.Function(b.ops)
 
 
+ALPHA158_CONFIG = {
+    "kbar": {},
+    "price": {"windows": [0], "feature": ["OPEN", "HIGH", "LOW", "VWAP"]},
+    "rolling": {"windows": [5, 10, 20, 30, 60]},
+}
+
+
+def alpha158_graph(K=None, AllData=None):
+    """The Alpha158 set as the reference's test_alpha158.py:17-38 builds it:
+    kbar features, OPEN/HIGH/LOW/VWAP of the day over close, and every
+    rolling feature over windows 5, 10, 20, 30, 60 (158 outputs). AllData:
+    the factor library's class (this package's by default)."""
+    K = K or native()
+    if AllData is None:
+        from ..predefined.Alpha158 import AllData
+    b = K.Builder()
+    with b:
+        pack = AllData(low=K.Input("low"), high=K.Input("high"), close=K.Input("close"), open=K.Input("open"),
+                       amount=K.Input("amount"), volume=K.Input("volume"))
+        cfg = {k: dict(v) for k, v in ALPHA158_CONFIG.items()}
+        cfg["price"]["feature"] = [(n, getattr(pack, n.lower())) for n in ALPHA158_CONFIG["price"]["feature"]]
+        fields, names = pack.build(cfg)
+        for v, k in zip(fields, names):
+            K.Output(v, k)
+    return K.Function(b.ops)
+
+
 def generic_cs_graph(K=None):
     """A generic cross-sectional op (body compiled as device code) between
     time-series stages: DiffWithWeightedSum(a - b, c) (the reference's

@@ -211,6 +211,12 @@ def ops_inputs(S, T, dtype, seed):
     return {"a": a, "b": b}
 
 
+def alpha158_ref_graph(K):
+    sys.path.insert(0, REF)
+    from KunQuant.predefined.Alpha158 import AllData
+    return wgraphs.alpha158_graph(K, AllData)
+
+
 def generic_inputs(S, T, dtype, seed):
     rng = np.random.Generator(np.random.PCG64(seed))
     d = {k: rng.random((T, S)).astype(dtype) for k in ("a", "b", "c")}
@@ -220,6 +226,11 @@ def generic_inputs(S, T, dtype, seed):
 
 CASES = [
     # name, graph factory, compile config, dtype, inputs(S, T), S, T, start, L
+    # the reference's own Alpha158 library, compiled as its test_alpha158.py:40-43 does
+    ("alpha158_f64", alpha158_ref_graph,
+     dict(dtype="double", blocking_len=4, partition_factor=4, input_layout="TS", output_layout="TS",
+          options={"opt_reduce": True, "fast_log": True}),
+     np.float64, lambda: wdata.ohlcv(16, 200, seed=158, dtype=np.float64, nan_frac=0.02), 16, 200, 0, 200),
     ("generic_f32", wgraphs.generic_cs_graph, dict(input_layout="TS", output_layout="TS"), np.float32,
      lambda: generic_inputs(40, 60, np.float32, 21), 40, 60, 0, 60),
     ("simple_f32", wgraphs.simple_graph, dict(input_layout="TS", output_layout="TS"), np.float32,

@@ -22,6 +22,10 @@ BATCH = {
     "rand_f32": (lambda: graphs.random_graph(100, 102), dict(input_layout="TS", output_layout="TS")),
     "rand_f64": (lambda: graphs.random_graph(40, 103), dict(dtype="double", input_layout="TS", output_layout="TS",
                                                             options={"no_fast_stat": "no_warn"})),
+    # the reference's Alpha158 library against this package's (same factors, same ops)
+    "alpha158_f64": (graphs.alpha158_graph, dict(dtype="double", blocking_len=4, partition_factor=4,
+                                                 input_layout="TS", output_layout="TS",
+                                                 options={"opt_reduce": True, "fast_log": True})),
     # generic cross-sectional op (DiffWithWeightedSum) between time-series stages
     "generic_f32": (graphs.generic_cs_graph, dict(input_layout="TS", output_layout="TS")),
 }

@@ -25,6 +25,10 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
+GRAPH_DESC = {"cfg2_f32": "seeded random graph", "cfg2_f64": "seeded random graph",
+              "alpha158_f64": "Alpha158 library"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -33,7 +37,7 @@ def main():
     ap.add_argument("--module", default="cfg2_f32")
     ap.add_argument("--stocks", type=int, default=8192)
     ap.add_argument("--days", type=int, default=1024)
-    ap.add_argument("--cpu-rows", type=int, default=128, help="rows of the CPU baseline sample")
+    ap.add_argument("--cpu-rows", type=int, default=1024, help="rows of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=("factors", "jobs", "serve", "precision"), default="factors")
     ap.add_argument("--sessions", type=int, default=256, help="--workload serve: concurrent sessions")
@@ -131,7 +135,7 @@ def main():
             else:
                 # cross-sectional: the input row read once, the output row written once
                 nbytes = sum(buf_bytes(b) for b in ins) + sum(buf_bytes(b) for b in outs_)
-                label = {1: "rank", 2: "scale"}[kind]
+                label = {1: "rank", 2: "scale", 3: "generic cross-sectional"}[kind]
             per_kernel.append((avg, i, j, label, nbytes))
             stage_ms[i] = stage_ms.get(i, 0.0) + avg
     ex.enable_profiling(False)
@@ -168,7 +172,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32" if np_dt == np.float32 else "f64",
         "data": "synthetic (seeded OHLCV, 10% NaN cells)",
-        "config": {"workload": f"{len(outputs)}-factor seeded random graph, {S} stocks x {T} days, "
+        "config": {"workload": f"{len(outputs)}-factor {GRAPH_DESC.get(args.module, 'graph')}, {S} stocks x {T} days, "
                                f"{'fp32' if esz == 4 else 'fp64'}, batch runGraph with HBM-resident inputs",
                    "module": args.module, "stocks": S, "days": T, "factors": len(outputs), "inputs": len(inputs),
                    "stages": len(stages)},

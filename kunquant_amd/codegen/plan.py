@@ -21,7 +21,9 @@ higher level than the one computing it is materialised in HBM as a
 independent components, launched side by side along the grid's z axis.
 
 Stages holding Accumulator webs (or anything the split cannot express) run
-as a single SEQ level: the whole stage walks its rows sequentially."""
+as a single SEQ level: every row in order, the level split into the
+independent op groups of the stage (row-local values of the inputs are
+recomputed by each group that reads them)."""
 from typing import Dict, List, Optional, Set
 
 from ..Op import ConstantOp, ForeachBackWindow, Input, OpBase, Output, ReductionOp
@@ -81,8 +83,74 @@ def _units(ops: List[OpBase]):
     return units, unit_of
 
 
+_NOT_REPLICABLE = REC_OPS | ACC_OPS | BOUND_OPS | {"IterValue"}
+
+
+def _replicable(ops: List[OpBase]) -> Set[OpBase]:
+    """Row-local values of the inputs (elementwise ops and lags over inputs
+    and constants, nothing stateful; a windowed temporary of one, each group
+    then keeping its own history of it): cheap to recompute, so each group
+    that reads one gets its own copy instead of being joined to the others."""
+    rep: Set[OpBase] = set()
+    for op in ops:
+        if is_source(op) or op.get_parent() is not None:
+            continue
+        if isinstance(op, (ForeachBackWindow, ReductionOp, Output)) or op.__class__.__name__ in _NOT_REPLICABLE:
+            continue
+        if all(is_source(i) or i in rep for i in op.inputs):
+            rep.add(op)
+    return rep
+
+
 def _single_level(ops: List[OpBase], reason: str) -> StagePlan:
-    return StagePlan([Level(SEQ, [Component(list(ops), [])])], {}, reason)
+    """The whole stage as one sequential level, split into the independent
+    op groups it holds: ops joined by a value, a loop and its body or
+    reductions (an Accumulator and its SetAccumulator are joined by the
+    value edge between them) share a group; inputs, constants and row-local
+    values of them are replicated into every group reading them. Each group
+    walks its rows on its own lanes, so a stage of many independent running
+    statistics fills the GPU instead of one lane per stock."""
+    rep = _replicable(ops)
+    shared = lambda o: is_source(o) or o in rep
+    parent: Dict[OpBase, OpBase] = {}
+
+    def find(x):
+        while parent[x] is not x:
+            parent[x] = parent[parent[x]]
+            x = parent[x]
+        return x
+
+    def union(a, b):
+        ra, rb = find(a), find(b)
+        if ra is not rb:
+            parent[ra] = rb
+
+    members = [op for op in ops if not shared(op)]
+    for op in members:
+        parent[op] = op
+    for op in members:
+        for inp in op.inputs:
+            if not shared(inp):
+                union(op, inp)
+        if op.get_parent() is not None:
+            union(op, op.get_parent())
+        if isinstance(op, ReductionOp):
+            union(op, op.get_loop())
+    groups: Dict[OpBase, Set[OpBase]] = {}
+    for op in members:
+        groups.setdefault(find(op), set()).add(op)
+    pos = {op: i for i, op in enumerate(ops)}
+    comps = []
+    for g in sorted(groups.values(), key=lambda g: min(pos[o] for o in g)):
+        need = set(g)
+        todo = [i for o in g for i in o.inputs if shared(i)]
+        while todo:
+            x = todo.pop()
+            if x not in need:
+                need.add(x)
+                todo.extend(i for i in x.inputs if shared(i))
+        comps.append(Component([op for op in ops if op in need], []))
+    return StagePlan([Level(SEQ, comps)], {}, reason)
 
 
 def plan_stage(ops: List[OpBase], force_seq: bool = False) -> StagePlan:

@@ -27,9 +27,11 @@ CATALOG: Dict[str, Tuple[Callable, dict]] = {
                                                   options={"no_fast_stat": True})),
     "aligned_f32": (aligned_graph, dict(input_layout="TS", output_layout="TS", allow_unaligned=False)),
     "cs_f32": (cross_section_graph, dict(input_layout="TS", output_layout="TS")),
-    # the Alpha158 factor library as the reference's test compiles it
+    # the Alpha158 factor library as the reference's test compiles it (fast
+    # statistics on, as there; "no_warn" only silences the notice)
     "alpha158_f64": (alpha158_graph, dict(dtype="double", blocking_len=4, partition_factor=4, input_layout="TS",
-                                          output_layout="TS", options={"opt_reduce": True, "fast_log": True})),
+                                          output_layout="TS", options={"opt_reduce": True, "fast_log": True,
+                                                                       "no_fast_stat": "no_warn"})),
     # generic cross-sectional op (device-compiled body) in TS, STs and stream modules
     "generic_f32": (generic_cs_graph, dict(input_layout="TS", output_layout="TS")),
     "generic_f64_sts": (generic_cs_graph, dict(dtype="double", input_layout="STs", output_layout="STs",

@@ -104,8 +104,12 @@ def test_plan_invariants():
     """Every op lands in one component; recurrences only on sequential
     levels; values crossing levels are materialised by a lower level."""
     from kunquant_amd.Op import ConstantOp, ForeachBackWindow, Input
+    from kunquant_amd.codegen.plan import _replicable
     for name, st, plan in _batch_stage_plans():
         ops = st.func.ops
+        # a single-level stage replicates row-local values of its inputs
+        # into every group reading them
+        rep = _replicable(ops) if plan.fallback is not None else set()
         seen = {}
         for li, level in enumerate(plan.levels):
             for ci, comp in enumerate(level.comps):
@@ -113,7 +117,7 @@ def test_plan_invariants():
                 for op in comp.ops:
                     if isinstance(op, (Input, ConstantOp)):
                         continue
-                    assert op not in seen, (name, st.idx)
+                    assert op not in seen or op in rep, (name, st.idx)
                     seen[op] = li
                     cls = op.__class__.__name__
                     if cls in REC_OPS:

@@ -218,6 +218,16 @@ KUN_API int kunStreamPoolRead(KunStreamPoolHandle pool, int session, size_t buff
 KUN_API const void *kunStreamPoolRowDevice(KunStreamPoolHandle pool, int session, size_t buffer_handle);
 /* ticks submitted for the session so far; -1 on error */
 KUN_API long kunStreamPoolTicks(KunStreamPoolHandle pool, int session);
+/* A session resumed from a serialised stream state in the reference's byte
+ * layout (kunStreamSerializeStates of either library, same module and
+ * stocks). Return codes as kunStreamPoolAdmit; a bad blob is error 1 with
+ * the reference's messages. */
+KUN_API int kunStreamPoolAdmitState(KunStreamPoolHandle pool, size_t num_stocks, const char *state, size_t state_len,
+                                    int *session);
+/* The session's state in the reference's byte layout (waits for its cycle).
+ * *size = 0: *size receives the length only; otherwise buffer holds *size
+ * bytes and receives the blob, *size its length (error 1 if too small). */
+KUN_API int kunStreamPoolSerializeStates(KunStreamPoolHandle pool, int session, char *buffer, size_t *size);
 
 /* Module introspection. dtype: 0 float, 1 double. layouts: 0 STs, 1 TS, 2 STREAM. */
 KUN_API int kunModuleDtype(KunModuleHandle m);

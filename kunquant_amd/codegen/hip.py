@@ -1363,7 +1363,7 @@ def emit_module_parts(module: ModuleIR, nparts: int, lds_budget: int = None) -> 
             parts.append(f"  {{{kind}, {kern}, kun_{mid}_ph{p}_stages, {len(sts)}}},")
         parts.append("};")
         pool = f"{len(phases)}, kun_{mid}_phases, (const void *)&kun_{mid}_pinit"
-    wire = "0, nullptr, 0, nullptr"
+    wire = "0, nullptr, 0, nullptr, nullptr"
     if module.stream_mode:
         wbufs, wstates = wire_table(module, stream_ems)
         for j, wb in enumerate(wbufs):
@@ -1382,7 +1382,10 @@ def emit_module_parts(module: ModuleIR, nparts: int, lds_budget: int = None) -> 
                 parts.append(f"  {{{ws.kind}, {ws.stage}, {ws.tslot}, {ws.islot}, {ws.window}, "
                              f"{stream_ems[ws.stage].n_t}}},")
             parts.append("};")
-        wire = f"{len(wbufs)}, kun_{mid}_wbufs, {len(wstates)}, {wst}"
+        parts.append(f"static const int kun_{mid}_tslots[] = {{"
+                     + ", ".join(str(stream_ems[st.idx].n_t if st.idx in stream_ems else 0) for st in module.stages)
+                     + "};")
+        wire = f"{len(wbufs)}, kun_{mid}_wbufs, {len(wstates)}, {wst}, kun_{mid}_tslots"
     if module.stream_mode:
         parts.append(f"static const void *kun_{mid}_inits[] = {{"
                      + ", ".join(f"(const void *)&{ini}" if ini else "nullptr" for _, _, _, ini, _, _ in meta) + "};")

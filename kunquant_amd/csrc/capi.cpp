@@ -475,4 +475,35 @@ KUN_API long kunStreamPoolTicks(KunStreamPoolHandle pool, int session) {
     return t;
 }
 
+KUN_API int kunStreamPoolAdmitState(KunStreamPoolHandle pool, size_t num_stocks, const char *state, size_t state_len,
+                                    int *session) {
+    if (!state || !session) {
+        set_error("state and session must not be null");
+        return 1;
+    }
+    int sid = -1;
+    if (guarded([&] { sid = pool_admit_state((StreamPool *)pool, num_stocks, state, state_len); })) return 1;
+    if (sid < 0) return 2;
+    *session = sid;
+    return 0;
+}
+
+KUN_API int kunStreamPoolSerializeStates(KunStreamPoolHandle pool, int session, char *buffer, size_t *size) {
+    if (!size || (*size != 0 && !buffer)) {
+        set_error("size must not be null, and buffer not null when *size > 0");
+        return 1;
+    }
+    std::string blob;
+    if (guarded([&] { blob = pool_serialize((StreamPool *)pool, session); })) return 1;
+    const size_t cap = *size;
+    *size = blob.size();
+    if (cap == 0) return 0;
+    if (blob.size() > cap) {
+        set_error("buffer too small for the serialised state");
+        return 1;
+    }
+    memcpy(buffer, blob.data(), blob.size());
+    return 0;
+}
+
 } // extern "C"

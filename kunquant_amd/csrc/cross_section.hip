@@ -148,6 +148,117 @@ __device__ __forceinline__ void rank_row(const kun_cs::View &in, const kun_cs::V
     for (long s = threadIdx.x; s < S; s += BLOCK) dst[cs_at(out.layout, t + out.row0, s, S, out.rows, blk)] = res[s];
 }
 
+// Keys-only form: the keys alone are merge sorted (no stock payload), the
+// sorted keys go to LDS, and every element finds its tie group [a, b) by
+// binary search -- a = first sorted position holding its key, b = one past
+// the last -- then leaves straight from its registers, coalesced. a and b are
+// the group bounds of the pair form, so the ranks are the same numbers.
+template <typename T, int BLOCK, int ITEMS>
+__device__ __forceinline__ void rank_row_keys(const kun_cs::View &in, const kun_cs::View &out, long S, int blk,
+                                              const long t) {
+    using K = typename KeyOf<T>::K;
+    constexpr int N = BLOCK * ITEMS;
+    using Sort = rocprim::block_sort<K, BLOCK, ITEMS, rocprim::empty_type, rocprim::block_sort_algorithm::merge_sort>;
+    __shared__ union {
+        typename Sort::storage_type sort;
+        K skey[N];
+    } sh;
+    __shared__ int nvalid;
+    const T *src = (const T *)in.ptr;
+    T *dst = (T *)out.ptr;
+    const long rin = t + in.row0;
+    if (threadIdx.x == 0) nvalid = 0;
+    __syncthreads();
+    K keys[ITEMS], mine[ITEMS];
+    int cnt = 0;
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        const long s = threadIdx.x + (long)j * BLOCK;
+        K k = ~K(0);
+        if (s < S) {
+            T x = src[cs_at(in.layout, rin, s, S, in.rows, blk)];
+            k = rank_key(x);
+            cnt += (x == x) ? 1 : 0;
+        }
+        keys[j] = k;
+        mine[j] = k;
+    }
+    if (cnt) atomicAdd(&nvalid, cnt);
+    Sort().sort(keys, sh.sort, [](const K &a, const K &b) { return a < b; });
+    __syncthreads();
+    const int p0 = threadIdx.x * ITEMS;
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) sh.skey[p0 + j] = keys[j];
+    __syncthreads();
+    const int n = nvalid;
+    // lower bounds of every item, the ITEMS searches interleaved
+    int lo[ITEMS], len[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        lo[j] = 0;
+        len[j] = n;
+    }
+    for (int step = 0; step < 32; step++) {
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < ITEMS; j++) {
+            if (len[j] > 0) {
+                const int half = len[j] >> 1;
+                if (sh.skey[lo[j] + half] < mine[j]) {
+                    lo[j] += half + 1;
+                    len[j] -= half + 1;
+                } else {
+                    len[j] = half;
+                }
+                any = true;
+            }
+        }
+        if (!any) break;
+    }
+    // upper bounds, searched only for elements with ties
+    int hi[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        hi[j] = lo[j] + 1;
+        len[j] = (mine[j] != ~K(0) && hi[j] < n && sh.skey[hi[j]] == mine[j]) ? n - hi[j] : 0;
+    }
+    for (int step = 0; step < 32; step++) {
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < ITEMS; j++) {
+            if (len[j] > 0) {
+                const int half = len[j] >> 1;
+                if (!(mine[j] < sh.skey[hi[j] + half])) {
+                    hi[j] += half + 1;
+                    len[j] -= half + 1;
+                } else {
+                    len[j] = half;
+                }
+                any = true;
+            }
+        }
+        if (!any) break;
+    }
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        const long s = threadIdx.x + (long)j * BLOCK;
+        if (s >= S) continue;
+        T r = kun::qnan<T>();
+        if (mine[j] != ~K(0) && lo[j] < n) {
+            const int a = lo[j], b = hi[j];
+            const int64_t sum = ((int64_t)a + b + 1) * (int64_t)(b - a) / 2;
+            const T q = (b - a == 1) ? (T)sum : (T)sum / (T)(b - a);
+            r = q / (T)n;
+        }
+        dst[cs_at(out.layout, t + out.row0, s, S, out.rows, blk)] = r;
+    }
+}
+
+template <typename T, int BLOCK, int ITEMS>
+__global__ __launch_bounds__(BLOCK) void rank_rows_keys(kun_cs::View in, kun_cs::View out, long S, int blk) {
+    rank_row_keys<T, BLOCK, ITEMS>(in, out, S, blk, blockIdx.x);
+}
+
 // one workgroup per row of the views
 template <typename T, int BLOCK, int ITEMS, bool MERGE>
 __global__ __launch_bounds__(BLOCK) void rank_rows(kun_cs::View in, kun_cs::View out, long S, int blk) {

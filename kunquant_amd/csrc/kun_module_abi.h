@@ -8,7 +8,7 @@
 #ifndef KUN_MODULE_ABI_H
 #define KUN_MODULE_ABI_H
 
-#define KUN_MODULE_ABI_VERSION 0x4B554E08u
+#define KUN_MODULE_ABI_VERSION 0x4B554E09u
 
 enum { KUN_BUF_INPUT = 0, KUN_BUF_OUTPUT = 1, KUN_BUF_TEMP = 2 };
 enum { KUN_STAGE_TS = 0, KUN_STAGE_RANK = 1, KUN_STAGE_SCALE = 2, KUN_STAGE_GENERIC = 3 };
@@ -134,6 +134,7 @@ typedef struct {
     const KunWireBuf *wire_bufs;
     int nwire_state;
     const KunWireState *wire_states;
+    const int *stage_tslots; /* STREAM modules: T slots of each stage's state (its int32 slots follow) */
 } KunModuleDesc;
 
 #endif

@@ -19,6 +19,24 @@
 
 namespace kunrt {
 
+// Stream state in the reference's byte layout (stream.hip). WireHost: host
+// copies of one stream's rows ([S] per module buffer) and stage states
+// ([slot][S] T columns, then int32 columns); WireView: the stream's module,
+// stocks, tick count, tick count at its last resume and the per-wire-buffer
+// positions recorded then.
+struct WireHost {
+    std::vector<std::vector<char>> rows, state;
+};
+struct WireView {
+    const KunModuleDesc *m;
+    size_t S;
+    long tick, tick0;
+    std::vector<uint64_t> *pos0;
+};
+void wire_serialize(const WireView &v, WireHost &h, std::string &out);
+// throws "Failed to read initial stream buffer" / "Failed to deserialize state buffer"
+void wire_deserialize(const WireView &v, WireHost &h, const char *p, const char *end);
+
 struct Error : std::runtime_error {
     using std::runtime_error::runtime_error;
 };
@@ -200,5 +218,7 @@ int pool_done(StreamPool *p, unsigned long long cycle, bool wait);
 const void *pool_row_device(StreamPool *p, int sid, size_t b);
 void pool_read(StreamPool *p, int sid, size_t b, void *host);
 long pool_ticks(StreamPool *p, int sid);
+int pool_admit_state(StreamPool *p, size_t S, const char *state, size_t len);
+std::string pool_serialize(StreamPool *p, int sid);
 
 } // namespace kunrt

@@ -81,11 +81,6 @@ static void launch_stage_kernel(const void *fn, StreamCtx *c, int si, long tick)
 }
 
 // ---- reference wire format ---------------------------------------------
-// Host copies of a context's rows and stage states.
-struct WireHost {
-    std::vector<std::vector<char>> rows, state;
-};
-
 static void pull_state(StreamCtx *c, WireHost &h) {
     HIPCHECK(hipStreamSynchronize(c->ex->stream));
     h.rows.assign(c->m->nbuf, std::vector<char>(c->esz * c->S));
@@ -129,16 +124,16 @@ static const KunRingRef *longest_ring(const KunWireBuf &wb) {
 
 // IT: the integer type of the reference's per-lane NaN counters (int32 for
 // float modules, int64 for double)
-template <typename T, typename IT> static void wire_write(StreamCtx *c, WireHost &h, std::string &out) {
-    const KunModuleDesc *m = c->m;
-    const size_t S = c->S, B = (size_t)m->blocking, nblk = (S + B - 1) / B;
-    const long n = (long)c->tick;
+template <typename T, typename IT> static void wire_write(const WireView &c, WireHost &h, std::string &out) {
+    const KunModuleDesc *m = c.m;
+    const size_t S = c.S, B = (size_t)m->blocking, nblk = (S + B - 1) / B;
+    const long n = c.tick;
     std::vector<T> data;
     std::vector<uint64_t> posv(nblk);
     for (int j = 0; j < m->nwire_buf; j++) {
         const KunWireBuf &wb = m->wire_bufs[j];
         const long W = wb.window;
-        const long pos = W == 1 ? 0 : wmod((long)c->pos0[j] + n - (long)c->tick0, W);
+        const long pos = W == 1 ? 0 : wmod((long)(*c.pos0)[j] + n - c.tick0, W);
         const KunRingRef *r = longest_ring(wb);
         data.resize((size_t)W * S);
         for (long p = 0; p < W; p++) {
@@ -207,12 +202,12 @@ template <typename T, typename IT> static void wire_write(StreamCtx *c, WireHost
 }
 
 // Restores rows, rings and op state from a blob in the reference's layout;
-// `h` holds the freshly initialised state. The context restarts its tick
+// `h` holds the freshly initialised state. The owner restarts its tick
 // count at 0: age a of a value goes to ring slot (-1 - a) & (slots - 1).
 template <typename T, typename IT>
-static void wire_read(StreamCtx *c, WireHost &h, const char *p, const char *end) {
-    const KunModuleDesc *m = c->m;
-    const size_t S = c->S, B = (size_t)m->blocking, nblk = (S + B - 1) / B;
+static void wire_read(const WireView &c, WireHost &h, const char *p, const char *end) {
+    const KunModuleDesc *m = c.m;
+    const size_t S = c.S, B = (size_t)m->blocking, nblk = (S + B - 1) / B;
     auto take = [&](void *dst, size_t n) {
         if ((size_t)(end - p) < n) return false;
         memcpy(dst, p, n);
@@ -231,7 +226,7 @@ static void wire_read(StreamCtx *c, WireHost &h, const char *p, const char *end)
         for (size_t blk = 0; blk < nblk; blk++)
             if (posv[blk] != posv[0] || posv[blk] >= (uint64_t)W) throw Error("Failed to read initial stream buffer");
         const long pos = (long)posv[0];
-        c->pos0[j] = (uint64_t)pos;
+        (*c.pos0)[j] = (uint64_t)pos;
         auto age = [&](long a) { return data.data() + wmod(pos - 1 - a, W) * S; };
         if (wb.buf >= 0) memcpy(h.rows[wb.buf].data(), age(0), S * sizeof(T));
         for (int i = 0; i < wb.nrings; i++) {
@@ -299,8 +294,20 @@ static void wire_read(StreamCtx *c, WireHost &h, const char *p, const char *end)
             }
         }
     }
-    c->tick = 0;
-    c->tick0 = 0;
+}
+
+void wire_serialize(const WireView &v, WireHost &h, std::string &out) {
+    if (elem_size(v.m) == 8)
+        wire_write<double, int64_t>(v, h, out);
+    else
+        wire_write<float, int32_t>(v, h, out);
+}
+
+void wire_deserialize(const WireView &v, WireHost &h, const char *p, const char *end) {
+    if (elem_size(v.m) == 8)
+        wire_read<double, int64_t>(v, h, p, end);
+    else
+        wire_read<float, int32_t>(v, h, p, end);
 }
 
 StreamCtx *stream_create(std::shared_ptr<Executor> ex, const KunModuleDesc *m, size_t S,
@@ -350,11 +357,10 @@ StreamCtx *stream_create(std::shared_ptr<Executor> ex, const KunModuleDesc *m, s
         }
         WireHost h;
         pull_state(c.get(), h);
-        if (c->esz == 8)
-            wire_read<double, int64_t>(c.get(), h, blob.data(), blob.data() + blob.size());
-        else
-            wire_read<float, int32_t>(c.get(), h, blob.data(), blob.data() + blob.size());
+        wire_deserialize(WireView{m, S, 0, 0, &c->pos0}, h, blob.data(), blob.data() + blob.size());
         push_state(c.get(), h);
+        c->tick = 0;
+        c->tick0 = 0;
     }
     return c.release();
 }
@@ -436,10 +442,7 @@ std::string stream_serialize(StreamCtx *c) {
     WireHost h;
     pull_state(c, h);
     std::string out;
-    if (c->esz == 8)
-        wire_write<double, int64_t>(c, h, out);
-    else
-        wire_write<float, int32_t>(c, h, out);
+    wire_serialize(WireView{c->m, c->S, (long)c->tick, (long)c->tick0, &c->pos0}, h, out);
     return out;
 }
 

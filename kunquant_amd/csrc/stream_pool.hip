@@ -143,6 +143,10 @@ struct PoolSession {
     size_t s0 = 0, S = 0;
     long tick = 0;
     int slot = -1; // in-flight cycle slot, or -1
+    // reference wire positions recorded when the session resumed from a
+    // serialised state (see WireView)
+    long tick0 = 0;
+    std::vector<uint64_t> pos0;
 };
 
 struct PoolSlot {
@@ -321,6 +325,8 @@ int pool_admit(StreamPool *p, size_t S) {
     ss.S = S;
     ss.tick = 0;
     ss.slot = -1;
+    ss.tick0 = 0;
+    ss.pos0.assign(m->nwire_buf, 0);
     // NaN rows and initial state over the session's lanes (the pool's
     // streams touch other sessions only)
     std::lock_guard<std::mutex> g(p->ex->mu);
@@ -535,5 +541,85 @@ void pool_read(StreamPool *p, int sid, size_t b, void *host) {
 }
 
 long pool_ticks(StreamPool *p, int sid) { return live_session(p, sid).tick; }
+
+// One session's rows and stage states as a stream context of S stocks
+// would hold them (WireHost), copied out of / into the pool's [..][cap]
+// arrays with 2-D copies over the session's lanes.
+static void session_pull(StreamPool *p, const PoolSession &ss, WireHost &h) {
+    const KunModuleDesc *m = p->m;
+    const size_t S = ss.S, esz = p->esz, cap = p->cap;
+    h.rows.assign(m->nbuf, std::vector<char>(S * esz));
+    HIPCHECK(hipSetDevice(p->ex->device));
+    for (int b = 0; b < m->nbuf; b++)
+        HIPCHECK(hipMemcpy(h.rows[b].data(), (const char *)p->row(b) + ss.s0 * esz, S * esz, hipMemcpyDeviceToHost));
+    h.state.assign(m->nstage, std::vector<char>());
+    for (int si = 0; si < m->nstage; si++) {
+        const KunStageDesc &sd = m->stages[si];
+        if (sd.kind != KUN_STAGE_TS || !sd.state_bytes) continue;
+        const size_t nt = (size_t)m->stage_tslots[si], ni = ((size_t)sd.state_bytes - nt * esz) / 4;
+        h.state[si].resize((size_t)sd.state_bytes * S);
+        const char *base = (const char *)p->state[si];
+        if (nt)
+            HIPCHECK(hipMemcpy2D(h.state[si].data(), S * esz, base + ss.s0 * esz, cap * esz, S * esz, nt,
+                                 hipMemcpyDeviceToHost));
+        if (ni)
+            HIPCHECK(hipMemcpy2D(h.state[si].data() + nt * S * esz, S * 4, base + nt * cap * esz + ss.s0 * 4, cap * 4,
+                                 S * 4, ni, hipMemcpyDeviceToHost));
+    }
+}
+
+static void session_push(StreamPool *p, const PoolSession &ss, const WireHost &h) {
+    const KunModuleDesc *m = p->m;
+    const size_t S = ss.S, esz = p->esz, cap = p->cap;
+    HIPCHECK(hipSetDevice(p->ex->device));
+    for (int b = 0; b < m->nbuf; b++)
+        HIPCHECK(hipMemcpy((char *)p->row(b) + ss.s0 * esz, h.rows[b].data(), S * esz, hipMemcpyHostToDevice));
+    for (int si = 0; si < m->nstage; si++) {
+        const KunStageDesc &sd = m->stages[si];
+        if (sd.kind != KUN_STAGE_TS || !sd.state_bytes) continue;
+        const size_t nt = (size_t)m->stage_tslots[si], ni = ((size_t)sd.state_bytes - nt * esz) / 4;
+        char *base = (char *)p->state[si];
+        if (nt)
+            HIPCHECK(hipMemcpy2D(base + ss.s0 * esz, cap * esz, h.state[si].data(), S * esz, S * esz, nt,
+                                 hipMemcpyHostToDevice));
+        if (ni)
+            HIPCHECK(hipMemcpy2D(base + nt * cap * esz + ss.s0 * 4, cap * 4, h.state[si].data() + nt * S * esz, S * 4,
+                                 S * 4, ni, hipMemcpyHostToDevice));
+    }
+}
+
+// A session of S stocks resumed from a serialised stream state (the
+// reference's byte layout); -1 when the pool has no room. A bad blob
+// releases the range again and throws the reference's messages.
+int pool_admit_state(StreamPool *p, size_t S, const char *state, size_t len) {
+    if (!p->m->stage_tslots) throw Error("the module carries no stream-state table");
+    int sid = pool_admit(p, S);
+    if (sid < 0) return sid;
+    PoolSession &ss = p->sess[sid];
+    try {
+        WireHost h;
+        session_pull(p, ss, h);
+        wire_deserialize(WireView{p->m, S, 0, 0, &ss.pos0}, h, state, state + len);
+        session_push(p, ss, h);
+    } catch (...) {
+        pool_release(p, sid);
+        throw;
+    }
+    ss.tick = 0;
+    ss.tick0 = 0;
+    return sid;
+}
+
+// The session's state in the reference's byte layout (waits for its cycle)
+std::string pool_serialize(StreamPool *p, int sid) {
+    PoolSession &ss = live_session(p, sid);
+    if (!p->m->stage_tslots) throw Error("the module carries no stream-state table");
+    if (ss.slot >= 0) wait_slot(p, p->slots[ss.slot]);
+    WireHost h;
+    session_pull(p, ss, h);
+    std::string out;
+    wire_serialize(WireView{p->m, ss.S, ss.tick, ss.tick0, &ss.pos0}, h, out);
+    return out;
+}
 
 } // namespace kunrt

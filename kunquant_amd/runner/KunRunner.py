@@ -88,6 +88,8 @@ def _load():
     sig("kunStreamPoolRead", i, vp, i, sz, vp)
     sig("kunStreamPoolRowDevice", vp, vp, i, sz)
     sig("kunStreamPoolTicks", ctypes.c_long, vp, i)
+    sig("kunStreamPoolAdmitState", i, vp, sz, ctypes.c_char_p, sz, ctypes.POINTER(i))
+    sig("kunStreamPoolSerializeStates", i, vp, i, ctypes.c_char_p, ctypes.POINTER(sz))
     ip = ctypes.POINTER(ctypes.c_int)
     sig("kunModuleKernelInfo", i, vp, i, i, ip, ip, ip, ip, ip)
     sig("kunModuleDeviceBytes", sz, vp, sz, sz, ip)
@@ -673,10 +675,16 @@ class StreamPool:
     def free_stocks(self) -> int:
         return _load().kunStreamPoolFreeStocks(self._h)
 
-    def admit(self, num_stocks: int):
-        """Session id, or None when no contiguous range of the pool is free."""
+    def admit(self, num_stocks: int, state: bytes = None):
+        """Session id, or None when no contiguous range of the pool is free.
+        state: a serialised stream state (StreamContext.serializeStates of
+        this library or the reference's) the session resumes from."""
         sid = ctypes.c_int(-1)
-        rc = _load().kunStreamPoolAdmit(self._h, int(num_stocks), ctypes.byref(sid))
+        if state is not None:
+            state = bytes(state)
+            rc = _load().kunStreamPoolAdmitState(self._h, int(num_stocks), state, len(state), ctypes.byref(sid))
+        else:
+            rc = _load().kunStreamPoolAdmit(self._h, int(num_stocks), ctypes.byref(sid))
         if rc == 2:
             return None
         if rc:
@@ -750,6 +758,17 @@ class StreamPool:
 
     def ticks(self, sid: int) -> int:
         return _load().kunStreamPoolTicks(self._h, sid)
+
+    def serializeStates(self, sid: int) -> bytes:
+        """The session's state in the reference's stream-state byte layout."""
+        L = _load()
+        size = ctypes.c_size_t(0)
+        if L.kunStreamPoolSerializeStates(self._h, sid, None, ctypes.byref(size)):
+            raise RuntimeError(_err())
+        buf = ctypes.create_string_buffer(max(size.value, 1))
+        if L.kunStreamPoolSerializeStates(self._h, sid, buf, ctypes.byref(size)):
+            raise RuntimeError(_err())
+        return buf.raw[:size.value]
 
 
 class _Mem(ctypes.Structure):

@@ -101,3 +101,38 @@ def test_serving_graph_blobs(executor):
     for k in outs:
         assert _same(got[k][-3:], g[f"out_{k}"]), k
     _assert_same_state(m, S, st2.serializeStates(), g["state_end"].tobytes(), table)
+
+
+@pytest.mark.parametrize("name", ["stream_f32", "stream_f64"])
+def test_pool_session_blobs(executor, name):
+    """A pool session resumed from the reference's blob continues like the
+    reference's run and serialises to its later blob; a session fed the
+    rows from the start serialises to the reference's blob (next to a
+    second, unrelated session of the same pool)."""
+    g = np.load(os.path.join(GOLD, f"ref_{name}.npz"))
+    _, mod = catalog.load(name)
+    m = catalog.module_ir(name)
+    table = module_wire_table(m)
+    S, T = (int(x) for x in g["meta"])
+    a = g["in_a"].astype(mod.dtype)
+    outs = mod.getOutputNames()
+    pool = kr.StreamPool(executor, mod, capacity_stocks=256, max_sessions=4)
+    other = pool.admit(2 * S)
+    fresh = pool.admit(S)
+    resumed = pool.admit(S, state=g["state37"].tobytes())
+    noise = np.random.default_rng(5).random((T, 2 * S)).astype(mod.dtype)
+    for t in range(T):
+        batch = [(fresh, {"a": a[t]}), (other, {"a": noise[t]})]
+        if t >= 37:
+            batch.append((resumed, {"a": a[t]}))
+        pool.done(pool.submit(batch), wait=True)
+        if t >= 37:
+            for k in outs:
+                assert _same(pool.read(resumed, k), g[f"out_{k}"][t]), (k, t)
+        if t + 1 == 37:
+            _assert_same_state(m, S, pool.serializeStates(fresh), g["state37"].tobytes(), table)
+    _assert_same_state(m, S, pool.serializeStates(fresh), g["state60"].tobytes(), table)
+    _assert_same_state(m, S, pool.serializeStates(resumed), g["state60"].tobytes(), table)
+    with pytest.raises(RuntimeError, match="Failed to deserialize state buffer"):
+        pool.admit(S, state=g["state37"].tobytes()[:-10])
+    assert pool.admit(S) is not None   # the failed admission returned its lanes

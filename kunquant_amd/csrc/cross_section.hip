@@ -433,7 +433,16 @@ hipError_t launch_rank_t(const kun_cs::View &in, const kun_cs::View &out, long S
     if (S <= 1024) return launch_rank_n<T, 256, 4, false>(in, out, S, rows, blk, st);
     if (S <= 2048) return launch_rank_n<T, 1024, 2, true>(in, out, S, rows, blk, st);
     if (S <= 4096) return launch_rank_n<T, 1024, 4, true>(in, out, S, rows, blk, st);
-    if (S <= 8192) return launch_rank_n<T, 1024, 8, true>(in, out, S, rows, blk, st);
+    if (S <= 8192) {
+        // 64-bit keys: moving no stock payload through the merge pays for the
+        // binary searches (fp32: the pair form is faster)
+        if constexpr (sizeof(T) == 8) {
+            hipLaunchKernelGGL((rank_rows_keys<T, 1024, 8>), dim3((unsigned)rows), dim3(1024), 0, st, in, out, S, blk);
+            return hipGetLastError();
+        } else {
+            return launch_rank_n<T, 1024, 8, true>(in, out, S, rows, blk, st);
+        }
+    }
     if (S <= 16384) {
         // a 64-bit-key merge sort of 16384 pairs needs more LDS than a CU has
         if constexpr (sizeof(T) == 8)

@@ -79,8 +79,8 @@ Executor::Executor() : chunking(ChunkPolicy::from_env()) {
     HIPCHECK(hipGetDevice(&device));
     HIPCHECK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     // lanes for independent stages (default priority, like every stream here)
-    long nl = env_long("KUN_STAGE_LANES", 4);
-    nl = nl < 0 ? 0 : (nl > 8 ? 8 : nl);
+    long nl = env_long("KUN_STAGE_LANES", 6);
+    nl = nl < 0 ? 0 : (nl > 16 ? 16 : nl);
     HIPCHECK(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
     for (long l = 0; l < nl; l++) {
         hipStream_t s;

@@ -188,20 +188,73 @@ def main():
     }
 
     if not args.no_cpu_baseline:
-        from kunquant_amd.codegen.program import program_text
-        from oracle.kun_oracle import OracleProgram
         rows = min(args.cpu_rows, T)
         cores = min(16, os.cpu_count() or 1)
-        prog = OracleProgram(program_text(catalog.module_ir(args.module)))
         sample = {k: np.ascontiguousarray(d[k][:rows]) for k in inputs}
-        t0 = time.perf_counter()
-        prog.run(sample, S, rows, 0, rows, nthreads=cores)
-        cpu_s = time.perf_counter() - t0
-        result["cpu_baseline"] = {"value": len(outputs) * S * rows / cpu_s, "unit": "cells/s", "cores": cores,
-                                  "kind": "port",
-                                  "sample": f"same graph and data, first {rows} of {T} days, {S} stocks "
-                                            f"({cpu_s:.1f} s on {cores} host threads)"}
+        base, ref_outs = reference_cpu_baseline(args.module, sample, outputs, S, rows, cores, np_dt)
+        if ref_outs is not None:
+            # the reference's results against the device's on the same rows
+            same = sum(np.array_equal(outs[k][:rows].cpu().numpy(), ref_outs[k], equal_nan=True) for k in outputs)
+            base["device_outputs_identical"] = f"{same}/{len(outputs)} factors bit-identical (NaN payloads aside)"
+            del ref_outs
+        if base is None:
+            from kunquant_amd.codegen.program import program_text
+            from oracle.kun_oracle import OracleProgram
+            prog = OracleProgram(program_text(catalog.module_ir(args.module)))
+            t0 = time.perf_counter()
+            prog.run(sample, S, rows, 0, rows, nthreads=cores)
+            cpu_s = time.perf_counter() - t0
+            base = {"value": len(outputs) * S * rows / cpu_s, "unit": "cells/s", "cores": cores, "kind": "port",
+                    "sample": f"same graph and data, first {rows} of {T} days, {S} stocks "
+                              f"({cpu_s:.1f} s on {cores} host threads)"}
+        else:
+            base["sample"] = f"same graph and data, first {rows} of {T} days, {S} stocks " + base["sample"]
+        result["cpu_baseline"] = base
     print(json.dumps(result), flush=True)
+
+
+# modules whose graph and compile options the reference's library was built
+# with under oracle/_ref (oracle/build_ref.sh + tests/golden/gen_ref_vectors.py)
+REF_MODULES = {"cfg2_f32": "rand_f32", "alpha158_f64": "alpha158_f64"}
+
+
+def reference_cpu_baseline(module, sample, outputs, S, rows, cores, np_dt):
+    """The reference's own CPU library (built from its sources into
+    oracle/_ref, AVX2) running the same graph through its C API with a
+    `cores`-thread executor: (baseline entry, its outputs), or (None, None)
+    when that build is not present."""
+    import ctypes
+    import numpy as np
+    ref = REF_MODULES.get(module)
+    rdir = os.path.join(ROOT, "oracle", "_ref")
+    rt, so = os.path.join(rdir, "libKunRuntime.so"), os.path.join(rdir, f"{ref}.so") if ref else None
+    if not ref or not (os.path.exists(rt) and os.path.exists(so)):
+        return None, None
+    L = ctypes.CDLL(rt, mode=ctypes.RTLD_GLOBAL)
+    vp, sz, cp = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_char_p
+    for n, res, argt in [("kunCreateMultiThreadExecutor", vp, [ctypes.c_int]), ("kunLoadLibrary", vp, [cp]),
+                         ("kunGetModuleFromLibrary", vp, [vp, cp]), ("kunCreateBufferNameMap", vp, []),
+                         ("kunSetBufferNameMap", None, [vp, cp, vp]), ("kunDestoryBufferNameMap", None, [vp]),
+                         ("kunRunGraph", None, [vp, vp, vp, sz, sz, sz, sz])]:
+        f = getattr(L, n)
+        f.restype, f.argtypes = res, argt
+    ex = L.kunCreateMultiThreadExecutor(cores)
+    mod = L.kunGetModuleFromLibrary(L.kunLoadLibrary(so.encode()), ref.encode())
+    if not ex or not mod:
+        return None, None
+    outs = {k: np.empty((rows, S), dtype=np_dt) for k in outputs}
+    m = L.kunCreateBufferNameMap()
+    for k, v in list(sample.items()) + list(outs.items()):
+        L.kunSetBufferNameMap(m, k.encode(), v.ctypes.data)
+    warm = min(rows, 64)
+    L.kunRunGraph(ex, mod, m, S, rows, 0, warm)
+    t0 = time.perf_counter()
+    L.kunRunGraph(ex, mod, m, S, rows, 0, rows)
+    cpu_s = time.perf_counter() - t0
+    L.kunDestoryBufferNameMap(m)
+    return {"value": len(outputs) * S * rows / cpu_s, "unit": "cells/s", "cores": cores, "kind": "reference",
+            "sample": f"(the reference's CPU library built from its sources, AVX2, {cores}-thread executor: "
+                      f"{cpu_s:.2f} s)"}, outs
 
 
 def _checksums(outs):

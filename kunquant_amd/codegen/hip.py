@@ -103,6 +103,20 @@ def _window_reads(op) -> Optional[Tuple[OpBase, int]]:
     return None
 
 
+def _rank_copy_source(rank) -> Optional[OpBase]:
+    """The ReduceRank a rank's current value is a copy of, or None. The
+    reference builds a rank's state as `ReduceRank<T, N> v{cur}`
+    (CodegenCpp.py:246-270); when `cur` is itself a ReduceRank object --
+    directly or through Output / WindowedTempOutput / ReturnFirstValue, which
+    the reference emits as `auto v = w` copies -- C++ selects the copy
+    constructor, so the new rank compares against the other rank's value and
+    continues its less / equal counts (Ops.hpp:598-616)."""
+    src = rank.inputs[1]
+    while src.__class__.__name__ in ("Output", "WindowedTempOutput", "ReturnFirstValue"):
+        src = src.inputs[0]
+    return src if src.__class__.__name__ == "ReduceRank" else None
+
+
 _DEF = re.compile(r"\bconst\s+\w+\s+(\w+)\s*=")
 _ITER = re.compile(r"\biter\b")
 
@@ -355,8 +369,24 @@ class _Lowering:
                 sgn = "-" if name == "ReduceArgMax" else ""
                 L.append(f"{T} r{k}_v = {sgn}kun::inf<{T}>(), r{k}_i = 0;")
             elif name == "ReduceRank":
-                L.append(f"const {T} r{k}_cur = {self.v(r.inputs[1])};")
-                L.append(f"{T} r{k}_lt = 0, r{k}_eq = 0;")
+                src = _rank_copy_source(r)
+                if src is None:
+                    L.append(f"const {T} r{k}_cur = {self.v(r.inputs[1])};")
+                    L.append(f"{T} r{k}_lt = 0, r{k}_eq = 0;")
+                else:
+                    # the reference copy-constructs this rank from `src`
+                    # (see _rank_copy_source): it compares against src's
+                    # value and continues its counts; src's result is
+                    # lt + (eq + 1) / 2, so starting from result - 1/2 and
+                    # no equal count gives the same multiples of 1/2 exactly
+                    root = src
+                    while _rank_copy_source(root) is not None:
+                        root = _rank_copy_source(root)
+                    have = set(self.ops) | set(getattr(self, "ext", []))
+                    if src not in have or root.inputs[1] not in have:
+                        raise RuntimeError("HIP emitter: a rank of a rank whose values sit in another kernel")
+                    L.append(f"const {T} r{k}_cur = {self.v(root.inputs[1])};")
+                    L.append(f"{T} r{k}_lt = {self.v(src)} - {T}(0.5), r{k}_eq = 0;")
             elif name == "ReduceDecayLinear":
                 rw = r.attrs["window"]
                 L.append(f"const {T} r{k}_step = ({T})(1.0 / ((1.0 + {rw}) * {rw} / 2));")

@@ -6,7 +6,7 @@ import os
 from typing import Callable, Dict, Tuple
 
 from .. import PACKAGE_DIR
-from .graphs import (aligned_graph, alpha158_graph, cross_section_graph, generic_cs_graph, ops_graph, precision_graph, random_graph, simple_graph,
+from .graphs import (aligned_graph, alpha158_graph, cross_section_graph, generic_cs_graph, tsrank_chain_graph, ops_graph, precision_graph, random_graph, simple_graph,
                      stream_graph)
 
 MODULE_DIR = os.path.join(PACKAGE_DIR, "lib", "modules")
@@ -32,6 +32,8 @@ CATALOG: Dict[str, Tuple[Callable, dict]] = {
     "alpha158_f64": (alpha158_graph, dict(dtype="double", blocking_len=4, partition_factor=4, input_layout="TS",
                                           output_layout="TS", options={"opt_reduce": True, "fast_log": True,
                                                                        "no_fast_stat": "no_warn"})),
+    # rank of a rank (the reference's copy-constructed rank state)
+    "tsrank_chain_f32": (tsrank_chain_graph, dict(input_layout="TS", output_layout="TS")),
     # generic cross-sectional op (device-compiled body) in TS, STs and stream modules
     "generic_f32": (generic_cs_graph, dict(input_layout="TS", output_layout="TS")),
     "generic_f64_sts": (generic_cs_graph, dict(dtype="double", input_layout="STs", output_layout="STs",

@@ -274,6 +274,29 @@ def alpha158_graph(K=None, AllData=None):
     return K.Function(b.ops)
 
 
+def tsrank_chain_graph(K=None):
+    """A factor of the 100-factor graph (f022) with its intermediates as
+    outputs: TsArgMax over TsRank over TsRank over a rolling sum of the
+    cross-sectional rank of volume. The second TsRank's current value is the
+    first TsRank, the case where the reference's rank continues the other
+    rank's counts."""
+    K = K or native()
+    b = K.Builder()
+    with b:
+        r1 = K.Rank(K.Input("volume"))
+        s = K.Scale(r1)
+        r2 = K.Rank(s)
+        w = K.WindowedSum(r2, 10)
+        t60 = K.TsRank(w, 60)
+        t30 = K.TsRank(t60, 30)
+        K.Output(r2, "r2")
+        K.Output(w, "wsum")
+        K.Output(t60, "t60")
+        K.Output(t30, "t30")
+        K.Output(K.Abs(K.TsArgMax(t30, 20)), "f022")
+    return K.Function(b.ops)
+
+
 def generic_cs_graph(K=None):
     """A generic cross-sectional op (body compiled as device code) between
     time-series stages: DiffWithWeightedSum(a - b, c) (the reference's

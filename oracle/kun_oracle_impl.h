@@ -515,7 +515,26 @@ static void FN(eval_loop)(FN(Ctx) *c, const FN(StepEnv) *e, int L, long row) {
             case OP_ReduceMax: s0[l] = -INFINITY; break;
             case OP_ReduceArgMax: s0[l] = -INFINITY; s1[l] = 0; break;
             case OP_ReduceArgMin: s0[l] = INFINITY; s1[l] = 0; break;
-            case OP_ReduceRank: s0[l] = V(o->in[1])[l]; s1[l] = 0; s2[l] = 0; break;
+            case OP_ReduceRank: {
+                /* The reference constructs the rank state from its current
+                 * value (`ReduceRank<T, N> v{cur}`, CodegenCpp.py:246-270);
+                 * when that value is itself a ReduceRank (directly or
+                 * through Output / WindowedTempOutput / ReturnFirstValue
+                 * copies), C++ picks the copy constructor: the new rank
+                 * starts from the other's comparison value and counts
+                 * (Ops.hpp:598-616). */
+                int src = o->in[1];
+                while (sg->ops[src].kind == OP_Output || sg->ops[src].kind == OP_WindowedTempOutput ||
+                       sg->ops[src].kind == OP_ReturnFirstValue)
+                    src = sg->ops[src].in[0];
+                if (sg->ops[src].kind == OP_ReduceRank && c->st[src]) {
+                    const REAL *p0 = c->st[src];
+                    s0[l] = p0[l]; s1[l] = p0[LB + l]; s2[l] = p0[2 * LB + l];
+                } else {
+                    s0[l] = V(o->in[1])[l]; s1[l] = 0; s2[l] = 0;
+                }
+                break;
+            }
             case OP_ReduceDecayLinear:
                 s0[l] = (REAL)(1.0 / ((1.0 + o->window) * o->window / 2)); s1[l] = 0; break;
             default: break;

@@ -217,6 +217,10 @@ def alpha158_ref_graph(K):
     return wgraphs.alpha158_graph(K, AllData)
 
 
+# stocks whose columns the large-universe chain fixture keeps
+CHAIN_STOCKS = [128, 4774, 4775]
+
+
 def generic_inputs(S, T, dtype, seed):
     rng = np.random.Generator(np.random.PCG64(seed))
     d = {k: rng.random((T, S)).astype(dtype) for k in ("a", "b", "c")}
@@ -270,6 +274,19 @@ def main():
                             **{f"in_{k}": v for k, v in inputs.items()}, **{f"out_{k}": v for k, v in res.items()},
                             meta=np.array([S, T, start, L]))
         print("wrote", name, len(outs), "outputs")
+    if only and "tsrank_chain" in only:
+        # the bench's volume (8192 stocks x 1024 days, regenerated from its
+        # seed by the test); only three stocks' columns are stored
+        f = wgraphs.tsrank_chain_graph(K)
+        outs = output_names(f)
+        so = ref_compile(K, f, "tsrank_chain", dict(input_layout="TS", output_layout="TS"))
+        vol = wdata.ohlcv(8192, 1024, seed=2, dtype=np.float32, nan_frac=0.1)["volume"]
+        res = run_batch(so, "tsrank_chain", {"volume": np.ascontiguousarray(vol)}, outs, 8192, 1024, 0, 1024,
+                        np.float32)
+        np.savez_compressed(os.path.join(HERE, "ref_tsrank_chain_f32.npz"),
+                            **{f"out_{k}": v[:, CHAIN_STOCKS] for k, v in res.items()},
+                            stocks=np.array(CHAIN_STOCKS), meta=np.array([8192, 1024, 2]))
+        print("wrote tsrank_chain")
     if only:
         return
     rng = np.random.Generator(np.random.PCG64(11))

@@ -66,3 +66,21 @@ def test_oracle_matches_reference_stream(name):
         st.run()
         for k in outs:
             assert same_values(st.get(k), outs[k][i]), (k, i)
+
+
+def test_oracle_rank_of_rank_matches_reference():
+    """8192 stocks x 1024 days of the bench's volume (regenerated from its
+    seed), three stocks stored: TsRank over a TsRank continues the first
+    rank's counts, as the reference's copy-constructed rank state does
+    (values above the window length show it)."""
+    from kunquant_amd.workloads import data
+    z = np.load(os.path.join(GOLDEN, "ref_tsrank_chain_f32.npz"))
+    S, T, seed = (int(x) for x in z["meta"])
+    st = list(z["stocks"])
+    vol = data.ohlcv(S, T, seed=seed, dtype=np.float32, nan_frac=0.1)["volume"]
+    prog = OracleProgram(program_text(compileit(graphs.tsrank_chain_graph(), "chain", input_layout="TS",
+                                                output_layout="TS")))
+    got = prog.run({"volume": np.ascontiguousarray(vol)}, S, T, 0, T, nthreads=8)
+    for k in ("r2", "wsum", "t60", "t30", "f022"):
+        assert same_values(got[k][:, st], z[f"out_{k}"]), k
+    assert np.nanmax(z["out_t30"]) > 30.5

@@ -215,7 +215,7 @@ def main():
 
 # modules whose graph and compile options the reference's library was built
 # with under oracle/_ref (oracle/build_ref.sh + tests/golden/gen_ref_vectors.py)
-REF_MODULES = {"cfg2_f32": "rand_f32", "alpha158_f64": "alpha158_f64"}
+REF_MODULES = {"cfg2_f32": "rand_f32", "alpha158_f64": "alpha158_f64", "cfg2_f64": "cfg2_f64"}
 
 
 def reference_cpu_baseline(module, sample, outputs, S, rows, cores, np_dt):

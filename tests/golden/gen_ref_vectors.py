@@ -261,6 +261,16 @@ def main():
     if not os.path.exists(os.path.join(REF_OUT, "libKunRuntime.so")):
         subprocess.check_call(["bash", os.path.join(ROOT, "oracle", "build_ref.sh")])
     K = ref_namespace()
+    if "--bench-modules" in sys.argv:
+        # reference builds of bench graphs with no fixture: bench.py times
+        # them as its CPU baseline and compares their outputs with the
+        # device's (oracle/_ref only, nothing committed)
+        for name, factory, cfg in [
+                ("cfg2_f64", lambda K: wgraphs.random_graph(100, 102, K=K),
+                 dict(dtype="double", input_layout="TS", output_layout="TS", options={"no_fast_stat": "no_warn"}))]:
+            ref_compile(K, factory(K), name, dict(cfg))
+            print("built reference module", name)
+        return
     # --stream-only: rewrite the stream fixtures alone; --only=<case>: one batch case
     only = [a.split("=", 1)[1] for a in sys.argv if a.startswith("--only=")]
     batch_cases = [] if "--stream-only" in sys.argv else [c for c in CASES if not only or c[0] in only]

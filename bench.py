@@ -26,7 +26,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
 GRAPH_DESC = {"cfg2_f32": "seeded random graph", "cfg2_f64": "seeded random graph",
-              "alpha158_f64": "Alpha158 library"}
+              "alpha158_f64": "Alpha158 library", "prec_f32": "precision graph", "prec_f64": "precision graph"}
 
 
 def main():
@@ -215,7 +215,8 @@ def main():
 
 # modules whose graph and compile options the reference's library was built
 # with under oracle/_ref (oracle/build_ref.sh + tests/golden/gen_ref_vectors.py)
-REF_MODULES = {"cfg2_f32": "rand_f32", "alpha158_f64": "alpha158_f64", "cfg2_f64": "cfg2_f64"}
+REF_MODULES = {"cfg2_f32": "rand_f32", "alpha158_f64": "alpha158_f64", "cfg2_f64": "cfg2_f64",
+               "prec_f32": "prec_f32", "prec_f64": "prec_f64"}
 
 
 def reference_cpu_baseline(module, sample, outputs, S, rows, cores, np_dt):

@@ -267,7 +267,11 @@ def main():
         # device's (oracle/_ref only, nothing committed)
         for name, factory, cfg in [
                 ("cfg2_f64", lambda K: wgraphs.random_graph(100, 102, K=K),
-                 dict(dtype="double", input_layout="TS", output_layout="TS", options={"no_fast_stat": "no_warn"}))]:
+                 dict(dtype="double", input_layout="TS", output_layout="TS", options={"no_fast_stat": "no_warn"})),
+                ("prec_f32", lambda K: wgraphs.precision_graph(150, 3, K=K),
+                 dict(input_layout="TS", output_layout="TS")),
+                ("prec_f64", lambda K: wgraphs.precision_graph(150, 3, K=K),
+                 dict(dtype="double", input_layout="TS", output_layout="TS", options={"no_fast_stat": True}))]:
             ref_compile(K, factory(K), name, dict(cfg))
             print("built reference module", name)
         return

@@ -670,6 +670,7 @@ class CompEmitter(_Lowering):
         self.plan = plan
         self.kind = kind
         self.ext = list(comp.ext)
+        self.shadow = set(getattr(comp, "shadow", ()))
         self.ops: List[OpBase] = list(comp.ops)
         opset = set(self.ops)
         self.computed = {op for op in self.ops if not isinstance(op, (Input, ConstantOp))}
@@ -890,7 +891,7 @@ class CompEmitter(_Lowering):
         raise AssertionError("row sources are loaded at the top of the row")
 
     def output_lines(self, op, val: str) -> List[str]:
-        if not self.emit_stores:
+        if not self.emit_stores or op in self.shadow:
             return []
         b = self.m.buffer(op.attrs["name"])
         return [f"b{b.idx}[kun::at<{self.buf_layout(op)}>(i, s, A.S, A.L, A.blk)] = {val};"]
@@ -1135,7 +1136,7 @@ class LevelKernel:
             for op in c.srcs:
                 io.add(("r", op.attrs["name"]) if isinstance(op, Input) else ("m", c.idx[op]))
             for op in c.ops:
-                if isinstance(op, Output):
+                if isinstance(op, Output) and op not in c.shadow:
                     io.add(("w", op.attrs["name"]))
                 elif op in c.computed and op in plan.mat:
                     io.add(("m", c.idx[op]))
@@ -1147,7 +1148,7 @@ class LevelKernel:
         self.reads = sorted({module.buffer(op.attrs["name"]).idx for c in self.comps for op in c.ops
                              if isinstance(op, Input)})
         self.writes = sorted({module.buffer(op.attrs["name"]).idx for c in self.comps for op in c.ops
-                              if isinstance(op, Output)})
+                              if isinstance(op, Output) and op not in c.shadow})
         self.produced = {op for c in self.comps for op in c.computed if op in plan.mat}
         self.consumed = {op for c in self.comps for op in c.ext}
         self.mat_writes = sorted(plan.mat[op] for op in self.produced)

@@ -49,6 +49,9 @@ class Component:
     def __init__(self, ops: List[OpBase], ext: List[OpBase]):
         self.ops = ops
         self.ext = ext  # materialised values of lower levels read here
+        # Outputs this group computes only for its own reads of their
+        # history; another group stores them
+        self.shadow: Set[OpBase] = set()
 
 
 class Level:
@@ -111,7 +114,11 @@ def _single_level(ops: List[OpBase], reason: str) -> StagePlan:
     walks its rows on its own lanes, so a stage of many independent running
     statistics fills the GPU instead of one lane per stock."""
     rep = _replicable(ops)
-    shared = lambda o: is_source(o) or o in rep
+    # an Output of a row-local value whose history other ops read: every
+    # group reading it computes it (and keeps its history), one stores it
+    read_out = {i for o in ops for i in o.inputs if isinstance(i, Output)}
+    rep_out = {o for o in read_out if is_source(o.inputs[0]) or o.inputs[0] in rep}
+    shared = lambda o: is_source(o) or o in rep or o in rep_out
     parent: Dict[OpBase, OpBase] = {}
 
     def find(x):
@@ -150,6 +157,19 @@ def _single_level(ops: List[OpBase], reason: str) -> StagePlan:
                 need.add(x)
                 todo.extend(i for i in x.inputs if shared(i))
         comps.append(Component([op for op in ops if op in need], []))
+    for o in sorted(rep_out, key=lambda o: pos[o]):
+        holders = [c for c in comps if o in c.ops]
+        if not holders:
+            need, todo = {o}, list(o.inputs)
+            while todo:
+                x = todo.pop()
+                if x not in need:
+                    need.add(x)
+                    todo.extend(x.inputs)
+            comps.append(Component([op for op in ops if op in need], []))
+            continue
+        for c in holders[1:]:
+            c.shadow.add(o)
     return StagePlan([Level(SEQ, comps)], {}, reason)
 
 

@@ -110,6 +110,11 @@ def test_plan_invariants():
         # a single-level stage replicates row-local values of its inputs
         # into every group reading them
         rep = _replicable(ops) if plan.fallback is not None else set()
+        # Outputs of row-local values read back by several groups
+        rep |= {o for lv in plan.levels for c in lv.comps for o in c.shadow}
+        for lv in plan.levels:
+            for c in lv.comps:
+                rep |= {o for o in c.ops if any(o in cc.shadow for cc in lv.comps)}
         seen = {}
         for li, level in enumerate(plan.levels):
             for ci, comp in enumerate(level.comps):

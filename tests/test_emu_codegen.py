@@ -11,7 +11,7 @@ from oracle.kun_oracle import OracleProgram
 from emu.emu import EmuModule
 
 MODULES = ["simple_f32", "ops_f32", "ops_f64", "ops_f32_sts", "aligned_f32", "rand40_f64",
-           "streamgraph_batch_f64"]
+           "streamgraph_batch_f64", "alpha158_f64"]
 
 
 def _check_module(name, S, T, start, L, chunks, seed=3, nan_frac=0.05, lds_budget=None):

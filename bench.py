@@ -37,7 +37,8 @@ def main():
     ap.add_argument("--module", default="cfg2_f32")
     ap.add_argument("--stocks", type=int, default=8192)
     ap.add_argument("--days", type=int, default=1024)
-    ap.add_argument("--cpu-rows", type=int, default=1024, help="rows of the CPU baseline sample")
+    ap.add_argument("--cpu-rows", type=int, default=256, help="rows of the CPU baseline sample")
+    ap.add_argument("--no-e2e", action="store_true", help="skip the host-buffer (PCIe-inclusive) runGraph timing")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", choices=("factors", "jobs", "serve", "precision"), default="factors")
     ap.add_argument("--sessions", type=int, default=256, help="--workload serve: concurrent sessions")
@@ -50,6 +51,7 @@ def main():
     ap.add_argument("--jobs", type=int, default=64, help="jobs of --workload jobs")
     ap.add_argument("--no-host-baseline", action="store_true",
                     help="--workload jobs: skip the sequential host-buffer runGraph pass")
+    ap.add_argument("--no-graph", action="store_true", help="--workload jobs: skip the batch as one HIP graph")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device-to-device three times (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -78,6 +80,7 @@ def main():
 
     import numpy as np
     import torch
+    from kunquant_amd.build import RUNTIME_LIB
     from kunquant_amd.runner import KunRunner as kr
     from kunquant_amd.workloads import catalog, data
 
@@ -141,18 +144,43 @@ def main():
     ex.enable_profiling(False)
     per_kernel.sort(reverse=True)
     kernel_sum = sum(p[0] for p in per_kernel)
-    dom_ms, dom_i, dom_j, dom_label, dom_bytes = per_kernel[0]
+    # The dominant kernel: the one kernel function with the most time summed
+    # over its launches. Every Rank stage of one row length launches the
+    # same radix-sort kernel; every Scale stage the same row-sum + division
+    # pair (timed together); each time-series stage kernel is a function of
+    # its own.
+    groups = {}
+    for avg, i, j, label, nbytes in per_kernel:
+        key = label if label in ("rank", "scale") else f"kun_{args.module}_s{i}_k{j}"
+        g = groups.setdefault(key, {"ms": 0.0, "launches": 0, "bytes": 0, "label": label, "stage": i, "kern": j})
+        g["ms"] += avg
+        g["launches"] += 1
+        g["bytes"] += nbytes
+    dom_key = max(groups, key=lambda k: groups[k]["ms"])
+    dom = groups[dom_key]
+    dom_ms = dom["ms"] / dom["launches"]
+    dom_bytes = dom["bytes"] / dom["launches"]
     achieved = dom_bytes / (dom_ms / 1e3) / 1e9
-    # HBM traffic of the dominant kernel, from rocprofv3 FETCH_SIZE / WRITE_SIZE
-    # passes scaled by the same counters on a plain copy (profiles/*_traffic.json)
-    dom_name = f"kun_{args.module}_s{dom_i}_k{dom_j}"
-    traffic = None
+    dom_name = {"rank": rank_kernel_name(np_dt, S), "scale": "scale_sums + scale_div"}.get(dom_key, dom_key)
+    # the longest single launch, for comparison
+    lo_ms, lo_i, lo_j, lo_label, lo_bytes = per_kernel[0]
+    # HBM traffic of the dominant kernel: rocprofv3 FETCH_SIZE / WRITE_SIZE
+    # passes scaled by the same counters on a plain copy, recorded with the
+    # sha256 of the library the kernel lives in; taken only when that library
+    # is the one benched
+    traffic, traffic_note = None, "no traffic record for this kernel"
     tpath = os.path.join(ROOT, "profiles", f"{args.module}_{S}x{T}_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            tk = json.load(f).get("kernels", {}).get(dom_name)
-        if tk:
+            rec = json.load(f)
+        tk = rec.get("kernels", {}).get(dom_name)
+        lib_path = RUNTIME_LIB if dom_key in ("rank", "scale") else catalog.library_path(args.module)
+        want = rec.get("runtime_sha256" if dom_key in ("rank", "scale") else "module_sha256")
+        if tk and want == sha256_file(lib_path):
             traffic = tk["fetch_bytes"] + tk["write_bytes"]
+            traffic_note = f"rocprofv3 counters recorded at commit {rec.get('commit')}"
+        elif tk:
+            traffic_note = "traffic record is of another build of this kernel's library"
     graph_bytes = (len(inputs) + len(outputs)) * S * T * esz
     graph_gbs = graph_bytes / (ms / 1e3) / 1e9
     by_kind = {}
@@ -177,9 +205,13 @@ def main():
                    "module": args.module, "stocks": S, "days": T, "factors": len(outputs), "inputs": len(inputs),
                    "stages": len(stages)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": f"stage {dom_i} kernel {dom_j} ({dom_label})", "kernel_ms": dom_ms,
-                     "algorithmic_bytes": dom_bytes},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_note,
+                     "kernel": f"{dom_name} ({dom['label']})", "launches_per_step": dom["launches"],
+                     "kernel_ms": dom_ms, "total_ms_per_step": dom["ms"], "algorithmic_bytes": dom_bytes},
+        "longest_launch": {"kernel": f"stage {lo_i} kernel {lo_j} ({lo_label})", "kernel_ms": lo_ms,
+                           "algorithmic_bytes": lo_bytes,
+                           "achieved_GBs": lo_bytes / (lo_ms / 1e3) / 1e9,
+                           "frac": lo_bytes / (lo_ms / 1e3) / 1e9 / HBM_PEAK_GBS},
         "graph": {"algorithmic_bytes": graph_bytes, "achieved_GBs": graph_gbs,
                   "frac_of_hbm_peak": graph_gbs / HBM_PEAK_GBS, "sum_kernel_ms": kernel_sum,
                   "kernel_ms_by_kind": {k: round(v, 3) for k, v in by_kind.items()},
@@ -187,9 +219,25 @@ def main():
                   "top_kernels_ms": [[p[1], p[2], p[3], round(p[0], 4)] for p in per_kernel[:8]]},
     }
 
+    # end to end: the reference's API on host buffers (inputs copied to HBM,
+    # outputs back, PCIe both ways), same workload; not the headline value
+    if not args.no_e2e:
+        host_in = {k: d[k] for k in inputs}
+        host_out = {k: np.empty((T, S), dtype=np_dt) for k in outputs}
+        kr.runGraph(ex, mod, host_in, 0, T, host_out)
+        e2e = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            kr.runGraph(ex, mod, host_in, 0, T, host_out)
+            e2e.append((time.perf_counter() - t0) * 1e3)
+        e2e_ms = sorted(e2e)[1]
+        result["end_to_end"] = {"ms": e2e_ms, "cells_per_s": cells / (e2e_ms / 1e3),
+                                "pcie_bytes": graph_bytes, "note": "runGraph on host buffers, median of 3"}
+        del host_out
+
     if not args.no_cpu_baseline:
         rows = min(args.cpu_rows, T)
-        cores = min(16, os.cpu_count() or 1)
+        cores = len(os.sched_getaffinity(0))
         sample = {k: np.ascontiguousarray(d[k][:rows]) for k in inputs}
         base, ref_outs = reference_cpu_baseline(args.module, sample, outputs, S, rows, cores, np_dt)
         if ref_outs is not None:
@@ -201,14 +249,19 @@ def main():
             from kunquant_amd.codegen.program import program_text
             from oracle.kun_oracle import OracleProgram
             prog = OracleProgram(program_text(catalog.module_ir(args.module)))
-            t0 = time.perf_counter()
-            prog.run(sample, S, rows, 0, rows, nthreads=cores)
-            cpu_s = time.perf_counter() - t0
+            prog.run(sample, S, min(rows, 64), 0, min(rows, 64), nthreads=cores)
+            runs = []
+            for _ in range(5):
+                t0 = time.perf_counter()
+                prog.run(sample, S, rows, 0, rows, nthreads=cores)
+                runs.append(time.perf_counter() - t0)
+            cpu_s = sorted(runs)[2]
             base = {"value": len(outputs) * S * rows / cpu_s, "unit": "cells/s", "cores": cores, "kind": "port",
                     "sample": f"same graph and data, first {rows} of {T} days, {S} stocks "
-                              f"({cpu_s:.1f} s on {cores} host threads)"}
+                              f"(median of 5 runs: {cpu_s:.2f} s on {cores} host threads)"}
         else:
             base["sample"] = f"same graph and data, first {rows} of {T} days, {S} stocks " + base["sample"]
+        base["host"] = host_cpu()
         result["cpu_baseline"] = base
     print(json.dumps(result), flush=True)
 
@@ -249,13 +302,51 @@ def reference_cpu_baseline(module, sample, outputs, S, rows, cores, np_dt):
         L.kunSetBufferNameMap(m, k.encode(), v.ctypes.data)
     warm = min(rows, 64)
     L.kunRunGraph(ex, mod, m, S, rows, 0, warm)
-    t0 = time.perf_counter()
-    L.kunRunGraph(ex, mod, m, S, rows, 0, rows)
-    cpu_s = time.perf_counter() - t0
+    runs = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        L.kunRunGraph(ex, mod, m, S, rows, 0, rows)
+        runs.append(time.perf_counter() - t0)
+    cpu_s = sorted(runs)[2]
     L.kunDestoryBufferNameMap(m)
     return {"value": len(outputs) * S * rows / cpu_s, "unit": "cells/s", "cores": cores, "kind": "reference",
             "sample": f"(the reference's CPU library built from its sources, AVX2, {cores}-thread executor: "
-                      f"{cpu_s:.2f} s)"}, outs
+                      f"median of 5 runs {cpu_s:.2f} s, runs {', '.join(f'{r:.2f}' for r in runs)} s)"}, outs
+
+
+def host_cpu():
+    """Cores and model of the host the CPU baseline ran on."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity_cores": len(os.sched_getaffinity(0)), "model": model}
+
+
+def sha256_file(path):
+    import hashlib
+    h = hashlib.sha256()
+    with open(path, "rb") as f:
+        for chunk in iter(lambda: f.read(1 << 20), b""):
+            h.update(chunk)
+    return h.hexdigest()
+
+
+def rank_kernel_name(np_dt, S):
+    """Name (as rocprofv3 prints it) of the Rank kernel for rows of S stocks."""
+    import numpy as np
+    t = "float" if np_dt == np.float32 else "double"
+    for lim, block, items in ((256, 256, 1), (512, 256, 2), (1024, 256, 4), (2048, 512, 4), (4096, 1024, 4),
+                              (8192, 1024, 8), (16384, 1024, 16)):
+        if S <= lim:
+            keys_only = "true" if (items == 16 and t == "double") else "false"
+            return f"rank_rows<{t}, {block}, {items}, {keys_only}>"
+    return "rank_rows"
 
 
 def _checksums(outs):
@@ -334,6 +425,23 @@ def run_jobs(args):
     ex.release_temps()
     print(f"jobs: batched {batch_ms:.1f} ms, results equal {same}", file=sys.stderr, flush=True)
 
+    # 4. the batch as one HIP graph, replayed with one launch per step
+    graph_ms, graph_same, build_s = None, None, None
+    if not args.no_graph:
+        for _, _, _, o in items:
+            for v in o.values():
+                v.fill_(0.0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g = kr.BatchGraph(ex, tuples)
+        build_s = time.perf_counter() - t0
+        graph_ms = timed(g.launch, args.steps, args.warmup)
+        graph_same = all(_checksums(o) == r for (_, _, _, o), r in zip(items, ref_sums))
+        del g
+        torch.cuda.synchronize()
+        print(f"jobs: graph {graph_ms:.1f} ms (built in {build_s:.2f} s), results equal {graph_same}",
+              file=sys.stderr, flush=True)
+
     # 5. sequential plain runGraph on host buffers (the reference's API: PCIe both ways)
     host_ms = None
     if not args.no_host_baseline:
@@ -365,7 +473,9 @@ def run_jobs(args):
                     "sequential_host_runGraph_ms": host_ms,
                     "speedup_vs_sequential_device": seq_ms / batch_ms,
                     "speedup_vs_sequential_host": host_ms / batch_ms if host_ms else None,
-                    "graph_capture": "switched off (kunCaptureGraphBatch crashed the process)",
+                    "graph_ms": graph_ms, "graph_build_s": build_s, "graph_results_equal_sequential": graph_same,
+                    "graph_speedup_vs_sequential_device": seq_ms / graph_ms if graph_ms else None,
+                    "graph_speedup_vs_sequential_host": host_ms / graph_ms if (graph_ms and host_ms) else None,
                     "autotune_s": tune_s,
                     "policies": {f"{p[0]}/{p[1]}/{p[2]}": n for p, n in picked.items()},
                     "results_equal_sequential": same},

@@ -91,9 +91,11 @@ KUN_API KunStreamContextHandle kunCreateStream(KunExecutorHandle exec, KunModule
 KUN_API KunStatus kunCreateStreamEx(KunExecutorHandle exec, KunModuleHandle m, size_t num_stocks,
                                     const KunStreamExtraArgs *extra_args,
                                     KunStreamContextHandle *out_handle);
-/* CApi.h:209 kunStreamSerializeStates: with KUN_INIT_MEMORY and a too small
- * (or NULL) buffer, *size is set to the required size and KUN_INIT_ERROR is
- * returned. */
+/* CApi.h:209 kunStreamSerializeStates (argument checks of CApi.cpp:140-164):
+ * with KUN_INIT_MEMORY, a NULL `size`, or a NULL buffer with *size != 0, is
+ * KUN_INVALID_ARGUMENT; a buffer smaller than the state (a NULL buffer with
+ * *size == 0 is the size query) sets *size to the required size and returns
+ * KUN_INIT_ERROR. */
 KUN_API KunStatus kunStreamSerializeStates(KunStreamContextHandle context,
                                            KunStateBufferKind dump_kind, char *path_or_buffer,
                                            size_t *size);
@@ -118,8 +120,9 @@ KUN_API int kunRunGraphEx(KunExecutorHandle exec, KunModuleHandle m, KunBufferNa
 /* Same computation on device-resident buffers: the name map holds device
  * pointers in the module's layouts. Kernels are enqueued on `hip_stream`
  * (NULL: the executor's stream); the call returns without synchronising.
- * Temporaries come from the executor's pool, so repeated calls with the same
- * shapes allocate nothing after the first. */
+ * Temporaries are kept per (module, shape), so repeated calls with the same
+ * shapes allocate nothing after the first; a call waits on its stream for
+ * the previous call that used the same temporaries, on whatever stream. */
 KUN_API int kunRunGraphDevice(KunExecutorHandle exec, KunModuleHandle m, KunBufferNameMapHandle buffers,
                               size_t num_stocks, size_t total_time, size_t cur_time, size_t length,
                               void *hip_stream);
@@ -145,17 +148,35 @@ KUN_API int kunRunGraphBatchDevice(KunExecutorHandle exec, int njobs, const KunM
                                    const KunBufferNameMapHandle *bufs, const size_t *num_stocks,
                                    const size_t *total_time, const size_t *cur_time, const size_t *length,
                                    void *hip_stream);
-/* The same batch captured once as a HIP graph (with temporaries of its own)
- * and replayed by kunLaunchGraphBatch: one launch for every kernel of every
- * job. The buffers' device pointers are baked in. NULL on failure.
- * Currently disabled: always returns NULL with an error message (capture
- * crashed the calling process on MI355X; cause not found yet). */
+/* The same batch as one HIP graph (with temporaries of its own), replayed
+ * by kunLaunchGraphBatch with one hipGraphLaunch for every kernel of every
+ * job. The graph is built node by node from the jobs' launch DAGs (no
+ * stream capture, no events inside it). The buffers' device pointers are
+ * baked in. NULL on failure (kunGetLastError). */
 KUN_API KunBatchGraphHandle kunCaptureGraphBatch(KunExecutorHandle exec, int njobs, const KunModuleHandle *mods,
                                                  const KunBufferNameMapHandle *bufs, const size_t *num_stocks,
                                                  const size_t *total_time, const size_t *cur_time,
                                                  const size_t *length);
+/* One replay on `hip_stream` (NULL: the executor's stream). A replay waits
+ * for the previous replay of the same graph, whatever its stream (they
+ * share the graph's temporaries). Returns without synchronising. */
 KUN_API int kunLaunchGraphBatch(KunBatchGraphHandle g, void *hip_stream);
+/* Waits for the last replay, then frees the graph and its temporaries. */
 KUN_API void kunDestroyGraphBatch(KunBatchGraphHandle g);
+/* Host-only description (no device calls) of the graph kunCaptureGraphBatch
+ * builds for these jobs, with `lanes` stage lanes in the launch plan: one
+ * line per graph node "<id> kernel|empty <job> <stage> <kernel> <grid>
+ * <block> : <dependency ids>", then one per launch-DAG node "dag <job>
+ * <node> <stage> <kernel> : <dependency nodes>". With a NULL buffer (or
+ * one smaller than the text) *size receives the bytes needed. */
+KUN_API int kunDescribeGraphBatch(int njobs, const KunModuleHandle *mods, const size_t *num_stocks,
+                                  const size_t *total_time, const size_t *cur_time, const size_t *length, int lanes,
+                                  char *buffer, size_t *size);
+/* Load serial of the library the module handle was obtained from: every
+ * successful kunLoadLibrary gets a new one. The executor's caches (device
+ * temporaries, chunking, stage order, timings) key by (module, serial), so a
+ * library loaded where an unloaded one was never meets the old entries. */
+KUN_API unsigned long long kunModuleLoadSerial(KunModuleHandle m);
 /* Blocks until the executor's stream is idle. */
 KUN_API int kunExecutorSynchronize(KunExecutorHandle exec);
 

@@ -2,6 +2,7 @@
 // Handles follow the reference's conventions (cpp/Kun/CApi.cpp:11-189):
 // executors and libraries are heap-boxed shared_ptrs, module handles are
 // borrowed descriptor pointers that stay valid while the library is loaded.
+#include <stdint.h>
 #include <string.h>
 
 #include <fstream>
@@ -85,11 +86,12 @@ KUN_API int kunExecutorSetChunkingFor(KunExecutorHandle exec, KunModuleHandle m,
         p.target_workgroups = target_workgroups;
         p.halo_factor = halo_factor;
         p.min_rows = min_rows;
-        ex.tuned[std::make_tuple((const void *)m, num_stocks, length)] = p;
+        const uint64_t serial = module_serial(unwrap_mod(m));
+        ex.tuned[std::make_tuple((const void *)m, serial, num_stocks, length)] = p;
         // drop cached temporaries of that module and shape: their layout changes
         for (auto it = ex.temps.begin(); it != ex.temps.end();) {
-            if (std::get<0>(it->first) == (const void *)m && std::get<1>(it->first) == num_stocks &&
-                std::get<2>(it->first) == length) {
+            if (std::get<0>(it->first) == (const void *)m && std::get<1>(it->first) == serial &&
+                std::get<2>(it->first) == num_stocks && std::get<3>(it->first) == length) {
                 free_device_temps(it->second);
                 it = ex.temps.erase(it);
             } else {
@@ -126,14 +128,7 @@ KUN_API KunBatchGraphHandle kunCaptureGraphBatch(KunExecutorHandle exec, int njo
                                                  const size_t *length) {
     CapturedBatch *cb = nullptr;
     guarded([&] {
-        // Switched off: capturing a batch of jobs ended the calling process
-        // with a segmentation fault on MI355X (ROCm 7.2), before or during
-        // hipStreamBeginCapture..hipGraphInstantiate, cause not found yet.
-        // Batches run through kunRunGraphBatchDevice meanwhile.
-        (void)capture_jobs;
-        throw Error("kunCaptureGraphBatch is disabled in this build (batch graph capture crashed the process); "
-                    "use kunRunGraphBatchDevice");
-        cb = capture_jobs(**unwrap_exec(exec), make_jobs(njobs, mods, bufs, num_stocks, total_time, cur_time, length));
+        cb = capture_jobs(*unwrap_exec(exec), make_jobs(njobs, mods, bufs, num_stocks, total_time, cur_time, length));
     });
     return cb;
 }
@@ -142,8 +137,34 @@ KUN_API int kunLaunchGraphBatch(KunBatchGraphHandle g, void *hip_stream) {
     return guarded([&] {
         CapturedBatch *cb = (CapturedBatch *)g;
         if (!cb) throw Error("null batch graph");
-        HIPCHECK(hipSetDevice(cb->device));
-        HIPCHECK(hipGraphLaunch(cb->exec, (hipStream_t)hip_stream));
+        launch_captured(*cb, (hipStream_t)hip_stream);
+    });
+}
+
+KUN_API int kunDescribeGraphBatch(int njobs, const KunModuleHandle *mods, const size_t *num_stocks,
+                                  const size_t *total_time, const size_t *cur_time, const size_t *length, int lanes,
+                                  char *buffer, size_t *size) {
+    return guarded([&] {
+        if (!size) throw Error("size is required");
+        if (njobs < 1) throw Error("a batch needs at least one job");
+        // every non-temporary buffer bound to a placeholder address
+        std::vector<NameMap> maps(njobs);
+        std::vector<BatchJob> jobs(njobs);
+        for (int j = 0; j < njobs; j++) {
+            const KunModuleDesc *m = unwrap_mod(mods[j]);
+            if (!m) throw Error("null module");
+            for (int b = 0; b < m->nbuf; b++)
+                if (m->bufs[b].kind != KUN_BUF_TEMP) maps[j][m->bufs[b].name] = (void *)(uintptr_t)(4096 * (b + 1));
+            jobs[j] = BatchJob{m, &maps[j], num_stocks[j], total_time[j], cur_time[j], length[j]};
+        }
+        const std::string text = describe_jobs_graph(jobs, lanes);
+        if (!buffer || *size < text.size() + 1) {
+            *size = text.size() + 1;
+            if (buffer) throw Error("buffer too small");
+            return;
+        }
+        memcpy(buffer, text.c_str(), text.size() + 1);
+        *size = text.size() + 1;
     });
 }
 
@@ -167,6 +188,8 @@ KUN_API KunModuleHandle kunGetModuleFromLibrary(KunLibraryHandle lib, const char
 }
 
 KUN_API void kunUnloadLibrary(KunLibraryHandle ptr) { delete unwrap_lib(ptr); }
+
+KUN_API unsigned long long kunModuleLoadSerial(KunModuleHandle m) { return module_serial(unwrap_mod(m)); }
 
 KUN_API KunBufferNameMapHandle kunCreateBufferNameMap(void) { return new NameMap(); }
 KUN_API void kunDestoryBufferNameMap(KunBufferNameMapHandle ptr) { delete unwrap_map(ptr); }
@@ -268,10 +291,13 @@ KUN_API int kunExecutorStageTime(KunExecutorHandle exec, KunModuleHandle m, int 
         ex.collect_profile();
         *total_ms = 0.0;
         *count = 0;
+        const uint64_t serial = module_serial(unwrap_mod(m));
         for (auto &kv : ex.stage_ms) {
-            if (std::get<0>(kv.first) != (const void *)m || std::get<1>(kv.first) != stage) continue;
+            if (std::get<0>(kv.first) != (const void *)m || std::get<1>(kv.first) != serial ||
+                std::get<2>(kv.first) != stage)
+                continue;
             *total_ms += kv.second.first;
-            if (std::get<2>(kv.first) == 0) *count = kv.second.second;
+            if (std::get<3>(kv.first) == 0) *count = kv.second.second;
         }
     });
 }
@@ -282,7 +308,7 @@ KUN_API int kunExecutorKernelTime(KunExecutorHandle exec, KunModuleHandle m, int
         Executor &ex = **unwrap_exec(exec);
         std::lock_guard<std::mutex> g(ex.mu);
         ex.collect_profile();
-        auto it = ex.stage_ms.find(std::make_tuple((const void *)m, stage, kern));
+        auto it = ex.stage_ms.find(std::make_tuple((const void *)m, module_serial(unwrap_mod(m)), stage, kern));
         *total_ms = it == ex.stage_ms.end() ? 0.0 : it->second.first;
         *count = it == ex.stage_ms.end() ? 0 : it->second.second;
     });
@@ -326,6 +352,9 @@ KUN_API KunStreamContextHandle kunCreateStream(KunExecutorHandle exec, KunModule
 KUN_API KunStatus kunStreamSerializeStates(KunStreamContextHandle context, KunStateBufferKind dump_kind,
                                            char *path_or_buffer, size_t *size) {
     if (dump_kind != KUN_INIT_FILE && dump_kind != KUN_INIT_MEMORY) return KUN_INVALID_ARGUMENT;
+    // a memory dump needs `size`, and a buffer whenever *size is not zero
+    // (a zero size with no buffer is the size query)
+    if (dump_kind == KUN_INIT_MEMORY && (!size || (*size != 0 && !path_or_buffer))) return KUN_INVALID_ARGUMENT;
     std::string blob;
     if (guarded([&] { blob = stream_serialize((StreamCtx *)context); })) return KUN_INIT_ERROR;
     if (dump_kind == KUN_INIT_FILE) {
@@ -335,7 +364,6 @@ KUN_API KunStatus kunStreamSerializeStates(KunStreamContextHandle context, KunSt
         f.write(blob.data(), (std::streamsize)blob.size());
         return f ? KUN_SUCCESS : KUN_INIT_ERROR;
     }
-    if (!size) return KUN_INVALID_ARGUMENT;
     if (!path_or_buffer || *size < blob.size()) {
         *size = blob.size();
         return KUN_INIT_ERROR;

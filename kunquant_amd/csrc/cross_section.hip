@@ -4,18 +4,19 @@
 // sorted; a value's rank is the average 1-based position of its tie group
 // [lo, hi): sum = (lo + hi + 1) * (hi - lo) / 2 in integers, then
 // T(sum) / T(hi - lo) / T(n). -0 and +0 tie (std::sort / equal_range use
-// operator<). On the device a row's (key, stock) pairs are sorted with an
-// LDS block radix sort of order-preserving integer keys (-0 canonicalised to
-// +0, NaN to the largest key) and tie groups are read off the sorted order.
+// operator<). On the device a row's values become order-preserving integer
+// keys (-0 canonicalised to +0, NaN to the largest key) and are sorted with
+// (stock) payloads by an LSD radix sort in LDS written for 64-lane
+// wavefronts (below); tie groups are read off the sorted order.
 //
 // Scale (restates cpp/Kun/Scale.hpp:13-46): sum of |x| over non-NaN values
 // accumulated sequentially in stock order (bit-exact order), then
 // x / sum, or NaN when x == 0 and sum == 0.
 #include <hip/hip_runtime.h>
-#include <rocprim/block/block_radix_sort.hpp>
-#include <rocprim/block/block_scan.hpp>
-#include <rocprim/block/block_sort.hpp>
 #include <stdint.h>
+
+#include <tuple>
+#include <utility>
 
 #include "kun_cs.hpp"
 #include "kun_device.hpp"
@@ -41,236 +42,373 @@ __device__ __forceinline__ size_t cs_at(int layout, long row, long s, long S, lo
     return ((size_t)(s / blk) * rows + row) * blk + s % blk;
 }
 
-// One workgroup per row. (key, stock) pairs are radix sorted; in the sorted
-// order the tie group [a, b) of position p is found without searching:
-// a = the last group start at or before p (a block max-scan of group
-// starts), b = the end recorded by the group's last element. Ranks go to LDS
-// at their stock index and leave in one coalesced pass.
-template <typename K, int BLOCK, int ITEMS, bool MERGE> struct PairSort;
-template <typename K, int BLOCK, int ITEMS> struct PairSort<K, BLOCK, ITEMS, false> {
-    using Impl = rocprim::block_radix_sort<K, BLOCK, ITEMS, int>;
-    using storage_type = typename Impl::storage_type;
-    __device__ __forceinline__ void sort(K (&k)[ITEMS], int (&v)[ITEMS], storage_type &st) { Impl().sort(k, v, st); }
-};
-template <typename K, int BLOCK, int ITEMS> struct PairSort<K, BLOCK, ITEMS, true> {
-    using Impl = rocprim::block_sort<K, BLOCK, ITEMS, int, rocprim::block_sort_algorithm::merge_sort>;
-    using storage_type = typename Impl::storage_type;
-    __device__ __forceinline__ void sort(K (&k)[ITEMS], int (&v)[ITEMS], storage_type &st) {
-        Impl().sort(k, v, st, [](const K &a, const K &b) { return a < b; });
+// A kernel launch through a sink: the arguments converted to the kernel's
+// parameter types, one pointer to each.
+template <typename... P, size_t... I>
+hipError_t sink_launch_impl(kun_cs::Sink &sink, const void *k, dim3 g, dim3 b, std::index_sequence<I...>,
+                            std::tuple<P...> vals) {
+    void *ptrs[] = {(void *)&std::get<I>(vals)...};
+    return sink.launch(k, g, b, ptrs);
+}
+template <typename... P, typename... A>
+hipError_t sink_launch(kun_cs::Sink &sink, void (*k)(P...), dim3 g, dim3 b, A... a) {
+    static_assert(sizeof...(P) == sizeof...(A), "kernel argument count");
+    return sink_launch_impl<P...>(sink, (const void *)k, g, b, std::index_sequence_for<P...>{},
+                                  std::tuple<P...>((P)a...));
+}
+
+// ---- wavefront primitives (64 lanes) ------------------------------------
+
+// number of lanes below this one whose bit is set in `mask`
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// lanes of the wavefront holding the same 8-bit digit as this lane: one
+// ballot per digit bit, each lane keeping the lanes that agree on that bit
+__device__ __forceinline__ uint64_t same_digit(uint32_t d) {
+    uint64_t peers = ~0ull;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const bool set = (d >> b) & 1u;
+        const uint64_t bal = __ballot(set);
+        peers &= set ? bal : ~bal;
+    }
+    return peers;
+}
+
+template <typename V> __device__ __forceinline__ V wave_incl_sum(V v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const V u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+    }
+    return v;
+}
+
+template <typename V> __device__ __forceinline__ V wave_incl_max(V v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const V u = __shfl_up(v, o, 64);
+        if (lane >= o && u > v) v = u;
+    }
+    return v;
+}
+
+template <typename V> __device__ __forceinline__ V wave_sum(V v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+template <typename V> __device__ __forceinline__ V wave_and(V v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v &= __shfl_xor(v, o, 64);
+    return v;
+}
+template <typename V> __device__ __forceinline__ V wave_or(V v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
+    return v;
+}
+
+// ---- Rank: LSD radix sort of one row in LDS -----------------------------
+//
+// A workgroup of BLOCK lanes (NW = BLOCK / 64 wavefronts) holds N = BLOCK x
+// ITEMS keys in registers, wavefront w owning positions [w*64*ITEMS,
+// (w+1)*64*ITEMS): item j of lane l is position w*64*ITEMS + j*64 + l
+// (loads and stores of consecutive lanes are consecutive). Each pass sorts
+// stably by one 8-bit digit:
+//  1. ranking: per item, the lanes with the same digit are found by ballots
+//     (same_digit); the lowest such lane adds their count to the wavefront's
+//     own LDS histogram row, every lane's rank within its digit being the
+//     row's previous count plus its lanes_below;
+//  2. a block scan turns the histograms into starting positions, digit by
+//     digit and within a digit wavefront by wavefront -- the order the
+//     positions held -- so the pass is stable;
+//  3. keys (and their stocks) are scattered to LDS and read back.
+// Digit positions on which every key of the row agrees are skipped (their
+// stable pass would keep the order); the row's AND and OR of keys tell.
+// Pairs form: keys carry their stock (16 bits) and tie groups come from a
+// max-scan of group starts in the sorted order. Keys-only form (64-bit keys
+// of rows above 8192 stocks, where pairs would not fit in LDS): each lane
+// finds its own keys' tie groups by binary search in the sorted keys.
+template <typename T, int BLOCK, int ITEMS, bool KEYS_ONLY> struct RadixRank {
+    using K = typename KeyOf<T>::K;
+    static constexpr int N = BLOCK * ITEMS;
+    static constexpr int NW = BLOCK / 64;
+    static_assert(NW % 4 == 0 && NW <= 16, "BLOCK must be 256, 512 or 1024");
+    static_assert(KEYS_ONLY || N <= 65536, "stock payloads are 16-bit");
+    struct Smem {
+        K key[N];                        // keys; then group ends (int); then ranks by stock (T)
+        uint16_t idx[KEYS_ONLY ? 1 : N]; // stock of the key at each position
+        uint32_t hist[NW][256];          // per wavefront digit counts, then starting positions
+        uint32_t wv[NW];                 // per wavefront totals of the scans
+        K wand[NW], wor[NW];
+    };
+    static_assert(sizeof(K) >= sizeof(int) && sizeof(K) >= sizeof(T), "LDS aliasing");
+
+    // hist := exclusive prefix in (digit, wavefront) order. Thread t owns
+    // digit t / (NW/4) of wavefronts 4g .. 4g+3 (g = t % (NW/4)): thread
+    // order is that order, so one block scan of per-thread sums does it.
+    static __device__ __forceinline__ void scan_hist(Smem &sm) {
+        constexpr int G = NW / 4;
+        const int t = threadIdx.x, d = t / G, g = t % G, lane = t & 63, w = t >> 6;
+        uint32_t v[4], s = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            v[i] = sm.hist[4 * g + i][d];
+            s += v[i];
+        }
+        const uint32_t incl = wave_incl_sum(s);
+        if (lane == 63) sm.wv[w] = incl;
+        __syncthreads();
+        uint32_t run = incl - s;
+        for (int q = 0; q < w; q++) run += sm.wv[q];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            sm.hist[4 * g + i][d] = run;
+            run += v[i];
+        }
+    }
+
+    static __device__ __forceinline__ void row(Smem &sm, const kun_cs::View &in, const kun_cs::View &out, long S,
+                                               int blk, long t) {
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        const int base = w * 64 * ITEMS + lane;
+        const T *src = (const T *)in.ptr;
+        T *dst = (T *)out.ptr;
+        const long rin = t + in.row0;
+        K key[ITEMS];
+        int id[KEYS_ONLY ? 1 : ITEMS];
+        int cnt = 0;
+        K kand = ~K(0), kor = K(0);
+#pragma unroll
+        for (int j = 0; j < ITEMS; j++) {
+            const long s = base + j * 64;
+            K k = ~K(0);
+            if (s < S) {
+                const T x = src[cs_at(in.layout, rin, s, S, in.rows, blk)];
+                k = rank_key(x);
+                cnt += (x == x) ? 1 : 0;
+            }
+            key[j] = k;
+            if constexpr (!KEYS_ONLY) id[j] = (int)s;
+            kand &= k;
+            kor |= k;
+        }
+        cnt = wave_sum(cnt);
+        kand = wave_and(kand);
+        kor = wave_or(kor);
+        if (lane == 0) {
+            sm.wv[w] = (uint32_t)cnt;
+            sm.wand[w] = kand;
+            sm.wor[w] = kor;
+        }
+        __syncthreads();
+        int n = 0;
+#pragma unroll
+        for (int q = 0; q < NW; q++) {
+            n += (int)sm.wv[q];
+            kand &= sm.wand[q];
+            kor |= sm.wor[q];
+        }
+        const K diff = kand ^ kor; // bits on which the row's keys differ
+        __syncthreads();           // sm.wv is the scans' from here on
+        bool moved = false;
+        for (int p = 0; p < (int)sizeof(K); p++) {
+            const int sh = 8 * p;
+            if (((diff >> sh) & K(0xff)) == K(0)) continue; // uniform over the workgroup
+            moved = true;
+            uint32_t *hw = sm.hist[w];
+            // this wavefront's row only: its reads of the row in the
+            // previous pass ended before the last barrier
+            for (int q = lane; q < 256; q += 64) hw[q] = 0;
+            uint32_t loc[ITEMS];
+#pragma unroll
+            for (int j = 0; j < ITEMS; j++) {
+                const uint32_t d = (uint32_t)(key[j] >> sh) & 0xffu;
+                const uint64_t peers = same_digit(d);
+                const uint32_t below = lanes_below(peers);
+                const uint32_t prior = hw[d];
+                if (below == 0) hw[d] = prior + (uint32_t)__popcll(peers);
+                loc[j] = prior + below;
+            }
+            __syncthreads();
+            scan_hist(sm);
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < ITEMS; j++) {
+                const uint32_t pos = hw[(uint32_t)(key[j] >> sh) & 0xffu] + loc[j];
+                sm.key[pos] = key[j];
+                if constexpr (!KEYS_ONLY) sm.idx[pos] = (uint16_t)id[j];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < ITEMS; j++) {
+                key[j] = sm.key[base + j * 64];
+                if constexpr (!KEYS_ONLY) id[j] = sm.idx[base + j * 64];
+            }
+        }
+        if (!moved) {
+            // every key equal: the order is sorted as it stands
+#pragma unroll
+            for (int j = 0; j < ITEMS; j++) sm.key[base + j * 64] = key[j];
+        }
+        __syncthreads(); // the sorted keys are in sm.key
+        if constexpr (KEYS_ONLY) {
+            // this lane's own keys again (the registers now hold sorted ones)
+#pragma unroll
+            for (int j = 0; j < ITEMS; j++) {
+                const long s = base + j * 64;
+                key[j] = s < S ? rank_key(src[cs_at(in.layout, rin, s, S, in.rows, blk)]) : ~K(0);
+            }
+            rank_by_search(sm, key, n, dst, out, S, blk, t, base);
+        } else {
+            rank_by_groups(sm, key, id, n, dst, out, S, blk, t, base);
+        }
+    }
+
+    static __device__ __forceinline__ T rank_of(int a, int b, int n) {
+        const int64_t sum = ((int64_t)a + b + 1) * (int64_t)(b - a) / 2;
+        // x / 1 == x exactly: a group of one skips the first division
+        const T q = (b - a == 1) ? (T)sum : (T)sum / (T)(b - a);
+        return q / (T)n;
+    }
+
+    static __device__ __forceinline__ void rank_by_groups(Smem &sm, const K (&key)[ITEMS], const int (&id)[ITEMS],
+                                                          int n, T *dst, const kun_cs::View &out, long S, int blk,
+                                                          long t, int base) {
+        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+        // group starts: a(P) = the last P' <= P starting a group (a max-scan
+        // over positions in (wavefront, item, lane) order)
+        int a[ITEMS];
+        bool last[ITEMS];
+        int carry = 0;
+#pragma unroll
+        for (int j = 0; j < ITEMS; j++) {
+            const int P = base + j * 64;
+            const bool first = P == 0 || sm.key[P - 1] != key[j];
+            last[j] = P + 1 == N || sm.key[P + 1] != key[j];
+            int v = wave_incl_max(first ? P : 0);
+            v = v > carry ? v : carry;
+            a[j] = v;
+            carry = __shfl(v, 63, 64);
+        }
+        if (lane == 63) sm.wv[w] = (uint32_t)carry;
+        __syncthreads(); // also: every neighbour read of sm.key is done
+        int pre = 0;
+        for (int q = 0; q < w; q++) pre = (int)sm.wv[q] > pre ? (int)sm.wv[q] : pre;
+        int *endof = (int *)sm.key;
+#pragma unroll
+        for (int j = 0; j < ITEMS; j++) {
+            a[j] = a[j] > pre ? a[j] : pre;
+            const int P = base + j * 64;
+            if (P < n && last[j]) endof[a[j]] = P + 1;
+        }
+        __syncthreads();
+        T r[ITEMS];
+#pragma unroll
+        for (int j = 0; j < ITEMS; j++) {
+            const int P = base + j * 64;
+            r[j] = P < n ? rank_of(a[j], endof[a[j]], n) : kun::qnan<T>();
+        }
+        __syncthreads();
+        T *res = (T *)sm.key;
+#pragma unroll
+        for (int j = 0; j < ITEMS; j++)
+            if (id[j] < S) res[id[j]] = r[j];
+        __syncthreads();
+        for (long s = threadIdx.x; s < S; s += BLOCK) dst[cs_at(out.layout, t + out.row0, s, S, out.rows, blk)] = res[s];
+    }
+
+    static __device__ __forceinline__ void rank_by_search(Smem &sm, const K (&mine)[ITEMS], int n, T *dst,
+                                                          const kun_cs::View &out, long S, int blk, long t,
+                                                          int base) {
+        // groups of G keys at a time, their searches interleaved
+        constexpr int G = ITEMS < 4 ? ITEMS : 4;
+#pragma unroll
+        for (int j0 = 0; j0 < ITEMS; j0 += G) {
+            // lower bounds
+            int lo[G], len[G];
+#pragma unroll
+            for (int j = 0; j < G; j++) {
+                lo[j] = 0;
+                len[j] = n;
+            }
+            for (int step = 0; step < 32; step++) {
+                bool any = false;
+#pragma unroll
+                for (int j = 0; j < G; j++) {
+                    if (len[j] > 0) {
+                        const int half = len[j] >> 1;
+                        if (sm.key[lo[j] + half] < mine[j0 + j]) {
+                            lo[j] += half + 1;
+                            len[j] -= half + 1;
+                        } else {
+                            len[j] = half;
+                        }
+                        any = true;
+                    }
+                }
+                if (!any) break;
+            }
+            // upper bounds, searched only for keys with ties
+            int hi[G];
+#pragma unroll
+            for (int j = 0; j < G; j++) {
+                hi[j] = lo[j] + 1;
+                len[j] = (mine[j0 + j] != ~K(0) && hi[j] < n && sm.key[hi[j]] == mine[j0 + j]) ? n - hi[j] : 0;
+            }
+            for (int step = 0; step < 32; step++) {
+                bool any = false;
+#pragma unroll
+                for (int j = 0; j < G; j++) {
+                    if (len[j] > 0) {
+                        const int half = len[j] >> 1;
+                        if (!(mine[j0 + j] < sm.key[hi[j] + half])) {
+                            hi[j] += half + 1;
+                            len[j] -= half + 1;
+                        } else {
+                            len[j] = half;
+                        }
+                        any = true;
+                    }
+                }
+                if (!any) break;
+            }
+#pragma unroll
+            for (int j = 0; j < G; j++) {
+                const long s = base + (j0 + j) * 64;
+                if (s >= S) continue;
+                const T r = (mine[j0 + j] != ~K(0) && lo[j] < n) ? rank_of(lo[j], hi[j], n) : kun::qnan<T>();
+                dst[cs_at(out.layout, t + out.row0, s, S, out.rows, blk)] = r;
+            }
+        }
     }
 };
 
-// MERGE: rocPRIM's merge sort (faster than its radix sort for large rows);
-// either sort serves, since ranks only need every element inside its tie
-// group.
-template <typename T, int BLOCK, int ITEMS, bool MERGE>
+template <typename T, int BLOCK, int ITEMS, bool KEYS_ONLY>
 __device__ __forceinline__ void rank_row(const kun_cs::View &in, const kun_cs::View &out, long S, int blk,
                                          const long t) {
-    using K = typename KeyOf<T>::K;
-    constexpr int N = BLOCK * ITEMS;
-    using Sort = PairSort<K, BLOCK, ITEMS, MERGE>;
-    using Scan = rocprim::block_scan<int, BLOCK>;
-    static_assert(sizeof(T) <= sizeof(K) && sizeof(int) <= sizeof(K), "LDS aliasing");
-    __shared__ union {
-        typename Sort::storage_type sort;
-        K skey[N];  // sorted keys; then group ends (int); then ranks by stock (T)
-    } sh;
-    __shared__ typename Scan::storage_type scan;
-    __shared__ int nvalid;
-    int *endof = (int *)sh.skey;
-    T *res = (T *)sh.skey;
-    const T *src = (const T *)in.ptr;
-    T *dst = (T *)out.ptr;
-    const long rin = t + in.row0;
-    if (threadIdx.x == 0) nvalid = 0;
-    __syncthreads();
-    K keys[ITEMS];
-    int idx[ITEMS];
-    int cnt = 0;
-#pragma unroll
-    for (int j = 0; j < ITEMS; j++) {
-        long s = threadIdx.x + (long)j * BLOCK;
-        K k = ~K(0);
-        if (s < S) {
-            T x = src[cs_at(in.layout, rin, s, S, in.rows, blk)];
-            k = rank_key(x);
-            cnt += (x == x) ? 1 : 0;
-        }
-        keys[j] = k;
-        idx[j] = (int)s;
-    }
-    if (cnt) atomicAdd(&nvalid, cnt);
-    Sort().sort(keys, idx, sh.sort);
-    __syncthreads();
-    const int p0 = threadIdx.x * ITEMS;  // blocked: this thread holds sorted positions p0 .. p0+ITEMS-1
-#pragma unroll
-    for (int j = 0; j < ITEMS; j++) sh.skey[p0 + j] = keys[j];
-    __syncthreads();
-    const int n = nvalid;
-    int start[ITEMS];
-    bool last[ITEMS];
-    int run = -1;
-#pragma unroll
-    for (int j = 0; j < ITEMS; j++) {
-        const int p = p0 + j;
-        if (p == 0 || sh.skey[p - 1] != keys[j]) run = p;
-        start[j] = run;
-        last[j] = p < n && (p == n - 1 || sh.skey[p + 1] != keys[j]);
-    }
-    int prefix;
-    Scan().exclusive_scan(run, prefix, -1, scan, rocprim::maximum<int>());
-    __syncthreads();  // every read of the sorted keys is done; reuse their LDS
-#pragma unroll
-    for (int j = 0; j < ITEMS; j++) {
-        start[j] = start[j] > prefix ? start[j] : prefix;
-        if (last[j]) endof[start[j]] = p0 + j + 1;
-    }
-    __syncthreads();
-    T r[ITEMS];
-#pragma unroll
-    for (int j = 0; j < ITEMS; j++) {
-        const int p = p0 + j;
-        if (p < n) {
-            const int a = start[j], b = endof[a];
-            const int64_t sum = ((int64_t)a + b + 1) * (int64_t)(b - a) / 2;
-            // x / 1 == x exactly: a group of one skips the first division
-            const T q = (b - a == 1) ? (T)sum : (T)sum / (T)(b - a);
-            r[j] = q / (T)n;
-        } else {
-            r[j] = kun::qnan<T>();
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < ITEMS; j++)
-        if (idx[j] < S) res[idx[j]] = r[j];
-    __syncthreads();
-    for (long s = threadIdx.x; s < S; s += BLOCK) dst[cs_at(out.layout, t + out.row0, s, S, out.rows, blk)] = res[s];
-}
-
-// Keys-only form: the keys alone are merge sorted (no stock payload), the
-// sorted keys go to LDS, and every element finds its tie group [a, b) by
-// binary search -- a = first sorted position holding its key, b = one past
-// the last -- then leaves straight from its registers, coalesced. a and b are
-// the group bounds of the pair form, so the ranks are the same numbers.
-template <typename T, int BLOCK, int ITEMS>
-__device__ __forceinline__ void rank_row_keys(const kun_cs::View &in, const kun_cs::View &out, long S, int blk,
-                                              const long t) {
-    using K = typename KeyOf<T>::K;
-    constexpr int N = BLOCK * ITEMS;
-    using Sort = rocprim::block_sort<K, BLOCK, ITEMS, rocprim::empty_type, rocprim::block_sort_algorithm::merge_sort>;
-    __shared__ union {
-        typename Sort::storage_type sort;
-        K skey[N];
-    } sh;
-    __shared__ int nvalid;
-    const T *src = (const T *)in.ptr;
-    T *dst = (T *)out.ptr;
-    const long rin = t + in.row0;
-    if (threadIdx.x == 0) nvalid = 0;
-    __syncthreads();
-    K keys[ITEMS], mine[ITEMS];
-    int cnt = 0;
-#pragma unroll
-    for (int j = 0; j < ITEMS; j++) {
-        const long s = threadIdx.x + (long)j * BLOCK;
-        K k = ~K(0);
-        if (s < S) {
-            T x = src[cs_at(in.layout, rin, s, S, in.rows, blk)];
-            k = rank_key(x);
-            cnt += (x == x) ? 1 : 0;
-        }
-        keys[j] = k;
-        mine[j] = k;
-    }
-    if (cnt) atomicAdd(&nvalid, cnt);
-    Sort().sort(keys, sh.sort, [](const K &a, const K &b) { return a < b; });
-    __syncthreads();
-    const int p0 = threadIdx.x * ITEMS;
-#pragma unroll
-    for (int j = 0; j < ITEMS; j++) sh.skey[p0 + j] = keys[j];
-    __syncthreads();
-    const int n = nvalid;
-    // lower bounds of every item, the ITEMS searches interleaved
-    int lo[ITEMS], len[ITEMS];
-#pragma unroll
-    for (int j = 0; j < ITEMS; j++) {
-        lo[j] = 0;
-        len[j] = n;
-    }
-    for (int step = 0; step < 32; step++) {
-        bool any = false;
-#pragma unroll
-        for (int j = 0; j < ITEMS; j++) {
-            if (len[j] > 0) {
-                const int half = len[j] >> 1;
-                if (sh.skey[lo[j] + half] < mine[j]) {
-                    lo[j] += half + 1;
-                    len[j] -= half + 1;
-                } else {
-                    len[j] = half;
-                }
-                any = true;
-            }
-        }
-        if (!any) break;
-    }
-    // upper bounds, searched only for elements with ties
-    int hi[ITEMS];
-#pragma unroll
-    for (int j = 0; j < ITEMS; j++) {
-        hi[j] = lo[j] + 1;
-        len[j] = (mine[j] != ~K(0) && hi[j] < n && sh.skey[hi[j]] == mine[j]) ? n - hi[j] : 0;
-    }
-    for (int step = 0; step < 32; step++) {
-        bool any = false;
-#pragma unroll
-        for (int j = 0; j < ITEMS; j++) {
-            if (len[j] > 0) {
-                const int half = len[j] >> 1;
-                if (!(mine[j] < sh.skey[hi[j] + half])) {
-                    hi[j] += half + 1;
-                    len[j] -= half + 1;
-                } else {
-                    len[j] = half;
-                }
-                any = true;
-            }
-        }
-        if (!any) break;
-    }
-#pragma unroll
-    for (int j = 0; j < ITEMS; j++) {
-        const long s = threadIdx.x + (long)j * BLOCK;
-        if (s >= S) continue;
-        T r = kun::qnan<T>();
-        if (mine[j] != ~K(0) && lo[j] < n) {
-            const int a = lo[j], b = hi[j];
-            const int64_t sum = ((int64_t)a + b + 1) * (int64_t)(b - a) / 2;
-            const T q = (b - a == 1) ? (T)sum : (T)sum / (T)(b - a);
-            r = q / (T)n;
-        }
-        dst[cs_at(out.layout, t + out.row0, s, S, out.rows, blk)] = r;
-    }
-}
-
-template <typename T, int BLOCK, int ITEMS>
-__global__ __launch_bounds__(BLOCK) void rank_rows_keys(kun_cs::View in, kun_cs::View out, long S, int blk) {
-    rank_row_keys<T, BLOCK, ITEMS>(in, out, S, blk, blockIdx.x);
+    using R = RadixRank<T, BLOCK, ITEMS, KEYS_ONLY>;
+    __shared__ typename R::Smem sm;
+    R::row(sm, in, out, S, blk, t);
 }
 
 // one workgroup per row of the views
-template <typename T, int BLOCK, int ITEMS, bool MERGE>
+template <typename T, int BLOCK, int ITEMS, bool KEYS_ONLY>
 __global__ __launch_bounds__(BLOCK) void rank_rows(kun_cs::View in, kun_cs::View out, long S, int blk) {
-    rank_row<T, BLOCK, ITEMS, MERGE>(in, out, S, blk, blockIdx.x);
+    rank_row<T, BLOCK, ITEMS, KEYS_ONLY>(in, out, S, blk, blockIdx.x);
 }
 
 // one workgroup per segment (a contiguous run of stocks of one row)
-template <typename T, int BLOCK, int ITEMS, bool MERGE>
+template <typename T, int BLOCK, int ITEMS, bool KEYS_ONLY>
 __global__ __launch_bounds__(BLOCK) void rank_segs(const kun_cs::Seg *segs) {
     const kun_cs::Seg g = segs[blockIdx.x];
     const kun_cs::View in{const_cast<void *>(g.in), 2, 1, 0}, out{g.out, 2, 1, 0};
-    rank_row<T, BLOCK, ITEMS, MERGE>(in, out, g.n, 1, 0);
+    rank_row<T, BLOCK, ITEMS, KEYS_ONLY>(in, out, g.n, 1, 0);
 }
 
 // ROWS rows per workgroup; the sum of each row is one sequential chain (lane
@@ -391,71 +529,54 @@ template <typename T> __global__ __launch_bounds__(SCALE_BLOCK) void scale_segs(
 
 template <typename T, int LIN, int LOUT>
 hipError_t launch_scale_split(const kun_cs::View &in, const kun_cs::View &out, long S, long rows, int blk,
-                              hipStream_t st, T *sums) {
-    hipLaunchKernelGGL((scale_sums<T, LIN>), dim3((unsigned)((rows + SCALE_SUM_ROWS - 1) / SCALE_SUM_ROWS)),
-                       dim3(SCALE_BLOCK), 0, st, in, S, rows, blk, sums);
-    hipLaunchKernelGGL((scale_div<T, LIN, LOUT>), dim3((unsigned)((rows * S + 255) / 256)), dim3(256), 0, st, in, out,
-                       S, rows, blk, (const T *)sums);
-    return hipGetLastError();
+                              kun_cs::Sink &sink, T *sums) {
+    hipError_t e = sink_launch(sink, scale_sums<T, LIN>, dim3((unsigned)((rows + SCALE_SUM_ROWS - 1) / SCALE_SUM_ROWS)),
+                               dim3(SCALE_BLOCK), in, S, rows, blk, sums);
+    if (e != hipSuccess) return e;
+    return sink_launch(sink, scale_div<T, LIN, LOUT>, dim3((unsigned)((rows * S + 255) / 256)), dim3(256), in, out, S,
+                       rows, blk, (const T *)sums);
 }
 
 template <typename T>
 hipError_t launch_scale_t(const kun_cs::View &in, const kun_cs::View &out, long S, long rows, int blk,
-                          hipStream_t st, void *scratch) {
+                          kun_cs::Sink &sink, void *scratch) {
     if (scratch && rows > 1) {
         T *sums = (T *)scratch;
-        if (in.layout == 1 && out.layout == 1) return launch_scale_split<T, 1, 1>(in, out, S, rows, blk, st, sums);
-        return launch_scale_split<T, -1, -1>(in, out, S, rows, blk, st, sums);
+        if (in.layout == 1 && out.layout == 1) return launch_scale_split<T, 1, 1>(in, out, S, rows, blk, sink, sums);
+        return launch_scale_split<T, -1, -1>(in, out, S, rows, blk, sink, sums);
     }
-    const unsigned grid = (unsigned)((rows + SCALE_ROWS - 1) / SCALE_ROWS);
+    const dim3 grid((unsigned)((rows + SCALE_ROWS - 1) / SCALE_ROWS));
     if (in.layout == 1 && out.layout == 1)
-        hipLaunchKernelGGL((scale_rows<T, 1, 1>), dim3(grid), dim3(SCALE_BLOCK), 0, st, in, out, S, rows, blk);
-    else if (in.layout == 2 && out.layout == 2)
-        hipLaunchKernelGGL((scale_rows<T, 2, 2>), dim3(grid), dim3(SCALE_BLOCK), 0, st, in, out, S, rows, blk);
-    else
-        hipLaunchKernelGGL((scale_rows<T, -1, -1>), dim3(grid), dim3(SCALE_BLOCK), 0, st, in, out, S, rows, blk);
-    return hipGetLastError();
+        return sink_launch(sink, scale_rows<T, 1, 1>, grid, dim3(SCALE_BLOCK), in, out, S, rows, blk);
+    if (in.layout == 2 && out.layout == 2)
+        return sink_launch(sink, scale_rows<T, 2, 2>, grid, dim3(SCALE_BLOCK), in, out, S, rows, blk);
+    return sink_launch(sink, scale_rows<T, -1, -1>, grid, dim3(SCALE_BLOCK), in, out, S, rows, blk);
 }
 
-template <typename T, int BLOCK, int ITEMS, bool MERGE>
+template <typename T, int BLOCK, int ITEMS, bool KEYS_ONLY>
 hipError_t launch_rank_n(const kun_cs::View &in, const kun_cs::View &out, long S, long rows, int blk,
-                         hipStream_t st) {
-    hipLaunchKernelGGL((rank_rows<T, BLOCK, ITEMS, MERGE>), dim3((unsigned)rows), dim3(BLOCK), 0, st, in, out, S,
-                       blk);
-    return hipGetLastError();
+                         kun_cs::Sink &sink) {
+    return sink_launch(sink, rank_rows<T, BLOCK, ITEMS, KEYS_ONLY>, dim3((unsigned)rows), dim3(BLOCK), in, out, S, blk);
 }
 
+// Workgroup shape by row length: at most 1024 lanes, up to 16 keys per lane;
+// 64-bit keys of more than 8192 stocks sort without payloads (LDS).
 template <typename T>
 hipError_t launch_rank_t(const kun_cs::View &in, const kun_cs::View &out, long S, long rows, int blk,
-                         hipStream_t st) {
-    if (S <= 256) return launch_rank_n<T, 256, 1, false>(in, out, S, rows, blk, st);
-    if (S <= 512) return launch_rank_n<T, 256, 2, false>(in, out, S, rows, blk, st);
-    if (S <= 1024) return launch_rank_n<T, 256, 4, false>(in, out, S, rows, blk, st);
-    if (S <= 2048) return launch_rank_n<T, 1024, 2, true>(in, out, S, rows, blk, st);
-    if (S <= 4096) return launch_rank_n<T, 1024, 4, true>(in, out, S, rows, blk, st);
-    if (S <= 8192) {
-        // 64-bit keys: moving no stock payload through the merge pays for the
-        // binary searches (fp32: the pair form is faster)
-        if constexpr (sizeof(T) == 8) {
-            hipLaunchKernelGGL((rank_rows_keys<T, 1024, 8>), dim3((unsigned)rows), dim3(1024), 0, st, in, out, S, blk);
-            return hipGetLastError();
-        } else {
-            return launch_rank_n<T, 1024, 8, true>(in, out, S, rows, blk, st);
-        }
-    }
-    if (S <= 16384) {
-        // a 64-bit-key merge sort of 16384 pairs needs more LDS than a CU has
-        if constexpr (sizeof(T) == 8)
-            return launch_rank_n<T, 512, 32, false>(in, out, S, rows, blk, st);
-        else
-            return launch_rank_n<T, 1024, 16, true>(in, out, S, rows, blk, st);
-    }
+                         kun_cs::Sink &sink) {
+    if (S <= 256) return launch_rank_n<T, 256, 1, false>(in, out, S, rows, blk, sink);
+    if (S <= 512) return launch_rank_n<T, 256, 2, false>(in, out, S, rows, blk, sink);
+    if (S <= 1024) return launch_rank_n<T, 256, 4, false>(in, out, S, rows, blk, sink);
+    if (S <= 2048) return launch_rank_n<T, 512, 4, false>(in, out, S, rows, blk, sink);
+    if (S <= 4096) return launch_rank_n<T, 1024, 4, false>(in, out, S, rows, blk, sink);
+    if (S <= 8192) return launch_rank_n<T, 1024, 8, false>(in, out, S, rows, blk, sink);
+    if (S <= 16384) return launch_rank_n<T, 1024, 16, sizeof(T) == 8>(in, out, S, rows, blk, sink);
     return hipErrorInvalidValue;
 }
 
-template <typename T, int BLOCK, int ITEMS, bool MERGE>
+template <typename T, int BLOCK, int ITEMS, bool KEYS_ONLY>
 hipError_t launch_rank_segs_n(const kun_cs::Seg *segs, int nseg, hipStream_t st) {
-    hipLaunchKernelGGL((rank_segs<T, BLOCK, ITEMS, MERGE>), dim3((unsigned)nseg), dim3(BLOCK), 0, st, segs);
+    hipLaunchKernelGGL((rank_segs<T, BLOCK, ITEMS, KEYS_ONLY>), dim3((unsigned)nseg), dim3(BLOCK), 0, st, segs);
     return hipGetLastError();
 }
 
@@ -463,15 +584,10 @@ template <typename T> hipError_t launch_rank_segs_t(const kun_cs::Seg *segs, int
     if (maxS <= 256) return launch_rank_segs_n<T, 256, 1, false>(segs, nseg, st);
     if (maxS <= 512) return launch_rank_segs_n<T, 256, 2, false>(segs, nseg, st);
     if (maxS <= 1024) return launch_rank_segs_n<T, 256, 4, false>(segs, nseg, st);
-    if (maxS <= 2048) return launch_rank_segs_n<T, 1024, 2, true>(segs, nseg, st);
-    if (maxS <= 4096) return launch_rank_segs_n<T, 1024, 4, true>(segs, nseg, st);
-    if (maxS <= 8192) return launch_rank_segs_n<T, 1024, 8, true>(segs, nseg, st);
-    if (maxS <= 16384) {
-        if constexpr (sizeof(T) == 8)
-            return launch_rank_segs_n<T, 512, 32, false>(segs, nseg, st);
-        else
-            return launch_rank_segs_n<T, 1024, 16, true>(segs, nseg, st);
-    }
+    if (maxS <= 2048) return launch_rank_segs_n<T, 512, 4, false>(segs, nseg, st);
+    if (maxS <= 4096) return launch_rank_segs_n<T, 1024, 4, false>(segs, nseg, st);
+    if (maxS <= 8192) return launch_rank_segs_n<T, 1024, 8, false>(segs, nseg, st);
+    if (maxS <= 16384) return launch_rank_segs_n<T, 1024, 16, sizeof(T) == 8>(segs, nseg, st);
     return hipErrorInvalidValue;
 }
 
@@ -496,16 +612,28 @@ hipError_t launch_scale_segs(int dtype, const Seg *segs, int nseg, hipStream_t s
 
 long max_rank_stocks(int dtype) { return 16384; }
 
-hipError_t launch_rank(int dtype, const View &in, const View &out, long S, long rows, int blk, hipStream_t st) {
+hipError_t launch_rank(int dtype, const View &in, const View &out, long S, long rows, int blk, Sink &sink) {
     if (rows <= 0) return hipSuccess;
-    return dtype ? launch_rank_t<double>(in, out, S, rows, blk, st) : launch_rank_t<float>(in, out, S, rows, blk, st);
+    return dtype ? launch_rank_t<double>(in, out, S, rows, blk, sink)
+                 : launch_rank_t<float>(in, out, S, rows, blk, sink);
+}
+
+hipError_t launch_rank(int dtype, const View &in, const View &out, long S, long rows, int blk, hipStream_t st) {
+    StreamSink sink(st);
+    return launch_rank(dtype, in, out, S, rows, blk, sink);
+}
+
+hipError_t launch_scale(int dtype, const View &in, const View &out, long S, long rows, int blk, Sink &sink,
+                        void *scratch) {
+    if (rows <= 0) return hipSuccess;
+    return dtype ? launch_scale_t<double>(in, out, S, rows, blk, sink, scratch)
+                 : launch_scale_t<float>(in, out, S, rows, blk, sink, scratch);
 }
 
 hipError_t launch_scale(int dtype, const View &in, const View &out, long S, long rows, int blk, hipStream_t st,
                         void *scratch) {
-    if (rows <= 0) return hipSuccess;
-    return dtype ? launch_scale_t<double>(in, out, S, rows, blk, st, scratch)
-                 : launch_scale_t<float>(in, out, S, rows, blk, st, scratch);
+    StreamSink sink(st);
+    return launch_scale(dtype, in, out, S, rows, blk, sink, scratch);
 }
 
 } // namespace kun_cs

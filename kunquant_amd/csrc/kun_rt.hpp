@@ -104,6 +104,10 @@ struct DeviceTemps {
     std::vector<void *> bufs; // per module buffer: the temporary's address in the arena, else null
     char *arena = nullptr;
     BatchLayout lay;
+    // recorded on the caller's stream once a run's lanes have joined; the
+    // next run using this arena waits for it first, whatever its stream
+    hipEvent_t done = nullptr;
+    bool armed = false;
 };
 
 // An executor: one device, one HIP stream that calls are ordered on, a few
@@ -119,23 +123,24 @@ struct Executor {
     Pool pool;
     std::mutex mu;
     ChunkPolicy chunking;
-    // temporaries of device-resident runs, per (module, stocks, rows)
-    // (module, stocks, rows, job slot); slot 0: single runs, j + 1: job j of a batch
-    std::map<std::tuple<const void *, size_t, size_t, int>, DeviceTemps> temps;
-    // chunking chosen for one (module, stocks, rows), e.g. by the autotuner
-    std::map<std::tuple<const void *, size_t, size_t>, ChunkPolicy> tuned;
+    // temporaries of device-resident runs, per (module, load serial, stocks,
+    // rows, job slot); slot 0: single runs, j + 1: job j of a batch
+    std::map<std::tuple<const void *, uint64_t, size_t, size_t, int>, DeviceTemps> temps;
+    // chunking chosen for one (module, load serial, stocks, rows), e.g. by the autotuner
+    std::map<std::tuple<const void *, uint64_t, size_t, size_t>, ChunkPolicy> tuned;
     ChunkPolicy policy_for(const KunModuleDesc *m, size_t S, size_t L) const;
     // per-stage kernel timing (bench / diagnostics)
     bool profiling = false;
     struct Pending {
         const void *module;
+        uint64_t serial;
         int stage, kern;
         hipEvent_t start, stop;
     };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> event_pool;
-    // (module, stage, kernel of the stage) -> (milliseconds, launches)
-    std::map<std::tuple<const void *, int, int>, std::pair<double, long>> stage_ms;
+    // (module, load serial, stage, kernel of the stage) -> (milliseconds, launches)
+    std::map<std::tuple<const void *, uint64_t, int, int>, std::pair<double, long>> stage_ms;
     hipEvent_t take_event();
     void collect_profile();
     Executor();
@@ -146,10 +151,16 @@ struct Executor {
 struct Library {
     void *handle = nullptr;
     std::string path;
+    uint64_t serial = 0; // load serial: distinct for every successful load in the process
     static std::shared_ptr<Library> load(const char *path);
     const KunModuleDesc *get_module(const char *name);
     ~Library();
 };
+// Load serial of the library a module descriptor was last obtained from
+// (0 for descriptors that did not come through Library::get_module). A
+// library loaded after another was unloaded can place its descriptors at
+// the old addresses; every cache keyed by a descriptor also keys by this.
+uint64_t module_serial(const KunModuleDesc *m);
 
 std::string module_symbol(const std::string &name);
 const std::vector<int> &stage_order(const KunModuleDesc *m);
@@ -157,6 +168,14 @@ size_t elem_size(const KunModuleDesc *m);
 size_t buffer_elems(const KunModuleDesc *m, int b, size_t S, size_t T, size_t L);
 void run_graph_host(Executor &ex, const KunModuleDesc *m, const NameMap &bufs, size_t S, size_t T,
                     size_t start, size_t L);
+// argument checks of a batch run (throws)
+void check_batch_module(const KunModuleDesc *m, size_t S, size_t T, size_t start, size_t L);
+// every node of `lay` in launch order on one stream; dev: device pointers of
+// the module's buffers, arena: the memory `lay` describes; prof: per-kernel
+// event timing (or null)
+void enqueue_batch_serial(const KunModuleDesc *m, const std::vector<void *> &dev, const BatchLayout &lay,
+                          char *arena, size_t S, size_t T, size_t start, size_t L, hipStream_t st,
+                          Executor *prof = nullptr);
 void run_graph_device(Executor &ex, const KunModuleDesc *m, const NameMap &bufs, size_t S, size_t T,
                       size_t start, size_t L, hipStream_t st);
 
@@ -168,15 +187,34 @@ struct BatchJob {
 };
 void run_jobs_device(Executor &ex, const std::vector<BatchJob> &jobs, hipStream_t st);
 
-// a batch of jobs captured as a HIP graph, with temporaries of its own
+// a batch of jobs as one HIP graph, with temporaries of its own
 struct CapturedBatch {
     int device = 0;
+    std::shared_ptr<Executor> ex; // its stream is the default replay stream
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<DeviceTemps> temps;
+    // recorded after each replay; a replay on another stream waits for it
+    hipEvent_t done = nullptr;
+    bool armed = false;
     ~CapturedBatch();
 };
-CapturedBatch *capture_jobs(Executor &ex, const std::vector<BatchJob> &jobs);
+// Builds a graph node by node. Node handles are opaque; deps are handles
+// returned earlier by the same builder.
+struct GraphBuilder {
+    virtual hipError_t add_kernel(const void *func, dim3 grid, dim3 block, void **args,
+                                  const std::vector<void *> &deps, void **node) = 0;
+    virtual hipError_t add_empty(const std::vector<void *> &deps, void **node) = 0;
+    virtual void tag(int job, int stage, int kern) { (void)job, (void)stage, (void)kern; }
+    virtual ~GraphBuilder() = default;
+};
+void build_jobs_graph(GraphBuilder &gb, const std::vector<BatchJob> &jobs,
+                      const std::vector<const DeviceTemps *> &temps);
+CapturedBatch *capture_jobs(std::shared_ptr<Executor> ex, const std::vector<BatchJob> &jobs);
+void launch_captured(CapturedBatch &cb, hipStream_t st);
+// text description of the graph capture_jobs builds for these jobs (host
+// only): nodes with their dependencies, and the jobs' launch DAGs
+std::string describe_jobs_graph(const std::vector<BatchJob> &jobs, int lanes);
 DeviceTemps make_temps(Executor &ex, const KunModuleDesc *m, size_t S, size_t T, size_t L);
 void free_device_temps(DeviceTemps &t);
 

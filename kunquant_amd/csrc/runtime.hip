@@ -132,7 +132,7 @@ void Executor::collect_profile() {
         HIPCHECK(hipEventSynchronize(p.stop));
         float ms = 0;
         HIPCHECK(hipEventElapsedTime(&ms, p.start, p.stop));
-        auto &acc = stage_ms[std::make_tuple(p.module, p.stage, p.kern)];
+        auto &acc = stage_ms[std::make_tuple(p.module, p.serial, p.stage, p.kern)];
         acc.first += ms;
         acc.second += 1;
         event_pool.push_back(p.start);
@@ -152,6 +152,10 @@ Library::~Library() {
     if (handle) dlclose(handle);
 }
 
+static std::mutex g_serial_mu;
+static uint64_t g_next_serial = 0;
+static std::unordered_map<const KunModuleDesc *, uint64_t> g_module_serial;
+
 std::shared_ptr<Library> Library::load(const char *path) {
     void *h = dlopen(path, RTLD_NOW | RTLD_LOCAL);
     if (!h) {
@@ -161,22 +165,35 @@ std::shared_ptr<Library> Library::load(const char *path) {
     auto lib = std::make_shared<Library>();
     lib->handle = h;
     lib->path = path;
+    std::lock_guard<std::mutex> g(g_serial_mu);
+    lib->serial = ++g_next_serial;
     return lib;
 }
 
 const KunModuleDesc *Library::get_module(const char *name) {
     auto *m = (const KunModuleDesc *)dlsym(handle, module_symbol(name).c_str());
     if (m && m->abi != KUN_MODULE_ABI_VERSION) return nullptr;
+    if (m) {
+        std::lock_guard<std::mutex> g(g_serial_mu);
+        g_module_serial[m] = serial;
+    }
     return m;
 }
 
-// topological order of the stage DAG (cached per descriptor)
+uint64_t module_serial(const KunModuleDesc *m) {
+    std::lock_guard<std::mutex> g(g_serial_mu);
+    auto it = g_module_serial.find(m);
+    return it == g_module_serial.end() ? 0 : it->second;
+}
+
+// topological order of the stage DAG (cached per descriptor and load serial)
 static std::mutex g_order_mu;
-static std::unordered_map<const KunModuleDesc *, std::vector<int>> g_orders;
+static std::map<std::pair<const KunModuleDesc *, uint64_t>, std::vector<int>> g_orders;
 
 const std::vector<int> &stage_order(const KunModuleDesc *m) {
+    const auto key = std::make_pair(m, module_serial(m));
     std::lock_guard<std::mutex> g(g_order_mu);
-    auto it = g_orders.find(m);
+    auto it = g_orders.find(key);
     if (it != g_orders.end()) return it->second;
     std::vector<int> indeg(m->nstage), order;
     for (int i = 0; i < m->nstage; i++) indeg[i] = m->stages[i].ndeps;
@@ -189,7 +206,10 @@ const std::vector<int> &stage_order(const KunModuleDesc *m) {
                 if (m->stages[j].deps[d] == s && --indeg[j] == 0) order.push_back(j);
     }
     if ((int)order.size() != m->nstage) throw Error("stage graph of the module has a cycle");
-    return g_orders.emplace(m, std::move(order)).first->second;
+    // drop the orders of descriptors that earlier loads placed at this address
+    for (auto o = g_orders.begin(); o != g_orders.end();)
+        o = (o->first.first == m) ? g_orders.erase(o) : std::next(o);
+    return g_orders.emplace(key, std::move(order)).first->second;
 }
 
 static bool is_single_value(const KunBufferDesc &b) { return strncmp(b.name, "__init", 6) == 0; }
@@ -403,12 +423,14 @@ BatchLayout batch_layout(const KunModuleDesc *m, size_t S, size_t L, const Chunk
     return lay;
 }
 
-// Enqueue kernel `kj` of stage `si` (cross-sectional stages: kj = 0) on `st`.
+// Kernels of kernel `kj` of stage `si` (cross-sectional stages: kj = 0)
+// into `sink` (a stream, or a graph under construction). prof: per-kernel
+// event timing on stream `st` (stream sinks only).
 static void enqueue_node(const KunModuleDesc *m, int si, int kj, const std::vector<void *> &dev,
                          const BatchLayout &lay, char *arena, size_t S, size_t T, size_t start, size_t L,
-                         hipStream_t st, Executor *prof) {
+                         kun_cs::Sink &sink, hipStream_t st, Executor *prof) {
     const KunStageDesc &sd = m->stages[si];
-    Executor::Pending ev{m, si, kj, nullptr, nullptr};
+    Executor::Pending ev{m, prof ? module_serial(m) : 0, si, kj, nullptr, nullptr};
     if (prof) {
         ev.start = prof->take_event();
         ev.stop = prof->take_event();
@@ -437,7 +459,8 @@ static void enqueue_node(const KunModuleDesc *m, int si, int kj, const std::vect
         memcpy(a.zscr, kl.zscr, sizeof a.zscr);
         void *args[] = {&a};
         if (kl.blocks > 0x7fffffffUL) throw Error("batch too large for one launch");
-        HIPCHECK(hipLaunchKernel(kd.kernel, dim3((unsigned)kl.blocks), dim3(sd.block), args, 0, st));
+        hipError_t e = sink.launch(kd.kernel, dim3((unsigned)kl.blocks), dim3(sd.block), args);
+        if (e != hipSuccess) throw Error(std::string("stage kernel launch: ") + hipGetErrorString(e));
     } else if (sd.kind == KUN_STAGE_GENERIC) {
         kun::CsArgs a;
         memset(&a, 0, sizeof a);
@@ -457,7 +480,8 @@ static void enqueue_node(const KunModuleDesc *m, int si, int kj, const std::vect
         a.L = (long)L;
         a.blk = m->blocking;
         void *args[] = {&a};
-        HIPCHECK(hipLaunchKernel(sd.stream_kernel, dim3((unsigned)((L + 63) / 64)), dim3(64), args, 0, st));
+        hipError_t e = sink.launch(sd.stream_kernel, dim3((unsigned)((L + 63) / 64)), dim3(64), args);
+        if (e != hipSuccess) throw Error(std::string("cross-sectional kernel launch: ") + hipGetErrorString(e));
     } else {
         int ib = sd.in[0], ob = sd.out[0];
         kun_cs::View vi = cs_view(m, ib, dev[ib], T, start, L);
@@ -466,8 +490,8 @@ static void enqueue_node(const KunModuleDesc *m, int si, int kj, const std::vect
             throw Error("Rank over more than " + std::to_string(kun_cs::max_rank_stocks(m->dtype)) +
                         " stocks is not supported");
         hipError_t e = sd.kind == KUN_STAGE_RANK
-                           ? kun_cs::launch_rank(m->dtype, vi, vo, (long)S, (long)L, m->blocking, st)
-                           : kun_cs::launch_scale(m->dtype, vi, vo, (long)S, (long)L, m->blocking, st,
+                           ? kun_cs::launch_rank(m->dtype, vi, vo, (long)S, (long)L, m->blocking, sink)
+                           : kun_cs::launch_scale(m->dtype, vi, vo, (long)S, (long)L, m->blocking, sink,
                                                   arena + lay.cs_scratch[si]);
         if (e != hipSuccess) throw Error(std::string("cross-sectional kernel: ") + hipGetErrorString(e));
     }
@@ -475,6 +499,13 @@ static void enqueue_node(const KunModuleDesc *m, int si, int kj, const std::vect
         HIPCHECK(hipEventRecord(ev.stop, st));
         prof->pending.push_back(ev);
     }
+}
+
+static void enqueue_node(const KunModuleDesc *m, int si, int kj, const std::vector<void *> &dev,
+                         const BatchLayout &lay, char *arena, size_t S, size_t T, size_t start, size_t L,
+                         hipStream_t st, Executor *prof) {
+    kun_cs::StreamSink sink(st);
+    enqueue_node(m, si, kj, dev, lay, arena, S, T, start, L, sink, st, prof);
 }
 
 // Launch DAG of a batch run: one node per stage kernel (one per
@@ -553,13 +584,19 @@ void plan_nodes(const KunModuleDesc *m, size_t S, size_t L, int lanes, BatchLayo
 // streams) as planned by plan_nodes; a node waits through events for its
 // dependencies on other lanes, every lane waits for `st` at the start and
 // `st` waits for every lane at the end.
+void enqueue_batch_serial(const KunModuleDesc *m, const std::vector<void *> &dev, const BatchLayout &lay,
+                          char *arena, size_t S, size_t T, size_t start, size_t L, hipStream_t st, Executor *prof) {
+    if (L == 0) return;
+    for (const auto &nd : lay.nodes) enqueue_node(m, nd.stage, nd.kern, dev, lay, arena, S, T, start, L, st, prof);
+}
+
 void enqueue_batch(Executor &ex, const KunModuleDesc *m, const std::vector<void *> &dev, const BatchLayout &lay,
                    char *arena, size_t S, size_t T, size_t start, size_t L, hipStream_t st) {
     if (L == 0) return;
     Executor *prof = ex.profiling ? &ex : nullptr;
     const int nl = (int)ex.lanes.size();
     if (nl == 0) {
-        for (const auto &nd : lay.nodes) enqueue_node(m, nd.stage, nd.kern, dev, lay, arena, S, T, start, L, st, prof);
+        enqueue_batch_serial(m, dev, lay, arena, S, T, start, L, st, prof);
         return;
     }
     ex.ensure_stage_events((int)lay.nodes.size());
@@ -646,7 +683,7 @@ void run_graph_host(Executor &ex, const KunModuleDesc *m, const NameMap &bufs, s
 
 // Device-buffer run: temporaries from the pool (kept for reuse per shape).
 ChunkPolicy Executor::policy_for(const KunModuleDesc *m, size_t S, size_t L) const {
-    auto it = tuned.find(std::make_tuple((const void *)m, S, L));
+    auto it = tuned.find(std::make_tuple((const void *)m, module_serial(m), S, L));
     return it == tuned.end() ? chunking : it->second;
 }
 
@@ -655,7 +692,14 @@ DeviceTemps make_temps(Executor &ex, const KunModuleDesc *m, size_t S, size_t T,
     DeviceTemps t;
     t.lay = batch_layout(m, S, L, ex.policy_for(m, S, L), (int)ex.lanes.size());
     t.bufs.assign(m->nbuf, nullptr);
-    if (t.lay.bytes) HIPCHECK(hipMalloc((void **)&t.arena, t.lay.bytes));
+    HIPCHECK(hipEventCreateWithFlags(&t.done, hipEventDisableTiming));
+    if (t.lay.bytes) {
+        hipError_t e = hipMalloc((void **)&t.arena, t.lay.bytes);
+        if (e != hipSuccess) {
+            hipEventDestroy(t.done);
+            throw Error(std::string("hipMalloc of a batch arena failed: ") + hipGetErrorString(e));
+        }
+    }
     for (int b = 0; b < m->nbuf; b++)
         if (m->bufs[b].kind == KUN_BUF_TEMP) t.bufs[b] = t.arena + t.lay.buf_off[b];
     return t;
@@ -663,14 +707,39 @@ DeviceTemps make_temps(Executor &ex, const KunModuleDesc *m, size_t S, size_t T,
 
 void free_device_temps(DeviceTemps &t) {
     if (t.arena) hipFree(t.arena);
+    if (t.done) hipEventDestroy(t.done);
     t.bufs.clear();
     t.arena = nullptr;
+    t.done = nullptr;
+    t.armed = false;
 }
 
-// temporaries of a device-resident run, cached per (module, shape, job slot)
-static const DeviceTemps &cached_temps(Executor &ex, const KunModuleDesc *m, size_t S, size_t T, size_t L,
-                                       int slot) {
-    auto key = std::make_tuple((const void *)m, S, L, slot);
+// A run on `st` that uses arena `t`: wait until the previous run using it
+// (possibly on another stream) has finished with it ...
+static void arena_acquire(DeviceTemps &t, hipStream_t st) {
+    if (t.armed) HIPCHECK(hipStreamWaitEvent(st, t.done, 0));
+}
+// ... and mark this run's end on `st` (after its lanes joined into `st`)
+static void arena_release(DeviceTemps &t, hipStream_t st) {
+    HIPCHECK(hipEventRecord(t.done, st));
+    t.armed = true;
+}
+
+// temporaries of a device-resident run, cached per (module, load serial,
+// shape, job slot); entries of a descriptor from an earlier load at the same
+// address are freed first
+static DeviceTemps &cached_temps(Executor &ex, const KunModuleDesc *m, size_t S, size_t T, size_t L, int slot) {
+    const uint64_t serial = module_serial(m);
+    for (auto it = ex.temps.begin(); it != ex.temps.end();) {
+        if (std::get<0>(it->first) == (const void *)m && std::get<1>(it->first) != serial) {
+            HIPCHECK(hipDeviceSynchronize());
+            free_device_temps(it->second);
+            it = ex.temps.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    auto key = std::make_tuple((const void *)m, serial, S, L, slot);
     auto it = ex.temps.find(key);
     if (it == ex.temps.end()) it = ex.temps.emplace(key, make_temps(ex, m, S, T, L)).first;
     return it->second;
@@ -697,8 +766,11 @@ void run_graph_device(Executor &ex, const KunModuleDesc *m, const NameMap &bufs,
     std::lock_guard<std::mutex> g(ex.mu);
     HIPCHECK(hipSetDevice(ex.device));
     if (!st) st = ex.stream;
-    const DeviceTemps &temps = cached_temps(ex, m, S, T, L, 0);
-    enqueue_batch(ex, m, device_pointers(m, bufs, temps), temps.lay, temps.arena, S, T, start, L, st);
+    DeviceTemps &temps = cached_temps(ex, m, S, T, L, 0);
+    const std::vector<void *> dev = device_pointers(m, bufs, temps);
+    arena_acquire(temps, st);
+    enqueue_batch(ex, m, dev, temps.lay, temps.arena, S, T, start, L, st);
+    arena_release(temps, st);
 }
 
 // Several independent jobs enqueued together: their launch DAGs interleave
@@ -758,65 +830,192 @@ void run_jobs_device(Executor &ex, const std::vector<BatchJob> &jobs, hipStream_
     std::lock_guard<std::mutex> g(ex.mu);
     HIPCHECK(hipSetDevice(ex.device));
     if (!st) st = ex.stream;
-    std::vector<const DeviceTemps *> temps;
+    std::vector<DeviceTemps *> own;
     for (size_t j = 0; j < jobs.size(); j++)
-        temps.push_back(&cached_temps(ex, jobs[j].m, jobs[j].S, jobs[j].T, jobs[j].L, (int)j + 1));
+        own.push_back(&cached_temps(ex, jobs[j].m, jobs[j].S, jobs[j].T, jobs[j].L, (int)j + 1));
+    std::vector<const DeviceTemps *> temps(own.begin(), own.end());
+    for (auto *t : own) arena_acquire(*t, st);
     enqueue_jobs(ex, jobs, temps, st, true);
+    for (auto *t : own) arena_release(*t, st);
 }
 
-// KUN_TRACE_CAPTURE=1: report each phase of a batch capture on stderr
-static void capture_phase(const char *what, size_t n) {
-    static const bool on = env_long("KUN_TRACE_CAPTURE", 0) != 0;
-    if (on) {
-        fprintf(stderr, "[kun capture] %s (%zu)\n", what, n);
-        fflush(stderr);
+// ---------------------------------------------------------------------------
+// A batch of jobs as one HIP graph, built node by node (hipGraphAddKernelNode)
+// from the jobs' launch DAGs rather than captured from streams: no events,
+// no lanes, no stream capture. Each DAG node contributes its kernels, the
+// first depending on the last kernel of each of the node's dependencies,
+// later ones on the kernel before them; jobs are independent subgraphs.
+
+namespace {
+struct GraphSink : kun_cs::Sink {
+    GraphBuilder &gb;
+    std::vector<void *> deps; // dependencies of the next kernel
+    void *last = nullptr;     // last kernel added
+    explicit GraphSink(GraphBuilder &g) : gb(g) {}
+    hipError_t launch(const void *func, dim3 grid, dim3 block, void **args) override {
+        void *node = nullptr;
+        hipError_t e = gb.add_kernel(func, grid, block, args, deps, &node);
+        if (e == hipSuccess) {
+            deps.assign(1, node);
+            last = node;
+        }
+        return e;
+    }
+};
+
+struct HipGraphBuilder : GraphBuilder {
+    hipGraph_t g;
+    explicit HipGraphBuilder(hipGraph_t graph) : g(graph) {}
+    hipError_t add_kernel(const void *func, dim3 grid, dim3 block, void **args, const std::vector<void *> &deps,
+                          void **node) override {
+        hipKernelNodeParams p;
+        memset(&p, 0, sizeof p);
+        p.func = const_cast<void *>(func);
+        p.gridDim = grid;
+        p.blockDim = block;
+        p.kernelParams = args;
+        p.sharedMemBytes = 0;
+        p.extra = nullptr;
+        hipGraphNode_t n = nullptr;
+        hipError_t e = hipGraphAddKernelNode(&n, g, (const hipGraphNode_t *)deps.data(), deps.size(), &p);
+        *node = n;
+        return e;
+    }
+    hipError_t add_empty(const std::vector<void *> &deps, void **node) override {
+        hipGraphNode_t n = nullptr;
+        hipError_t e = hipGraphAddEmptyNode(&n, g, (const hipGraphNode_t *)deps.data(), deps.size());
+        *node = n;
+        return e;
+    }
+};
+} // namespace
+
+void build_jobs_graph(GraphBuilder &gb, const std::vector<BatchJob> &jobs, const std::vector<const DeviceTemps *> &temps) {
+    for (size_t j = 0; j < jobs.size(); j++) {
+        const BatchJob &jb = jobs[j];
+        if (jb.L == 0) continue;
+        const DeviceTemps &t = *temps[j];
+        const std::vector<void *> dev = device_pointers(jb.m, *jb.bufs, t);
+        std::vector<void *> out(t.lay.nodes.size(), nullptr);
+        for (size_t n = 0; n < t.lay.nodes.size(); n++) {
+            const auto &nd = t.lay.nodes[n];
+            GraphSink sink(gb);
+            for (int d : nd.deps) {
+                if (d < 0 || (size_t)d >= n || !out[d]) throw Error("launch DAG out of topological order");
+                sink.deps.push_back(out[d]);
+            }
+            gb.tag((int)j, nd.stage, nd.kern);
+            enqueue_node(jb.m, nd.stage, nd.kern, dev, t.lay, t.arena, jb.S, jb.T, jb.start, jb.L, sink, nullptr,
+                         nullptr);
+            if (!sink.last) {
+                hipError_t e = gb.add_empty(sink.deps, &sink.last);
+                if (e != hipSuccess) throw Error(std::string("graph node: ") + hipGetErrorString(e));
+            }
+            out[n] = sink.last;
+        }
     }
 }
 
-CapturedBatch *capture_jobs(Executor &ex, const std::vector<BatchJob> &jobs) {
+CapturedBatch *capture_jobs(std::shared_ptr<Executor> exp, const std::vector<BatchJob> &jobs) {
     check_jobs(jobs);
+    Executor &ex = *exp;
     std::lock_guard<std::mutex> g(ex.mu);
     HIPCHECK(hipSetDevice(ex.device));
     std::unique_ptr<CapturedBatch> cb(new CapturedBatch());
     cb->device = ex.device;
+    cb->ex = exp;
     cb->temps.reserve(jobs.size());
     std::vector<const DeviceTemps *> temps;
     for (const auto &jb : jobs) {
         cb->temps.push_back(make_temps(ex, jb.m, jb.S, jb.T, jb.L));
         temps.push_back(&cb->temps.back());
     }
-    hipStream_t st = ex.stream;
-    HIPCHECK(hipStreamSynchronize(st));
-    capture_phase("temporaries allocated, begin", jobs.size());
-    HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    try {
-        enqueue_jobs(ex, jobs, temps, st, false);
-        capture_phase("enqueued", ex.stage_done.size());
-    } catch (...) {
-        hipGraph_t broken = nullptr;
-        hipStreamEndCapture(st, &broken);
-        if (broken) hipGraphDestroy(broken);
-        throw;
-    }
-    HIPCHECK(hipStreamEndCapture(st, &cb->graph));
-    size_t nnodes = 0;
-    HIPCHECK(hipGraphGetNodes(cb->graph, nullptr, &nnodes));
-    capture_phase("captured, graph nodes", nnodes);
-    // hipGraphInstantiate writes the failing node and a log through its
-    // out-parameters when instantiation fails: give it both
-    hipGraphNode_t bad = nullptr;
-    char log[1024] = {0};
-    hipError_t e = hipGraphInstantiate(&cb->exec, cb->graph, &bad, log, sizeof log - 1);
-    if (e != hipSuccess)
-        throw Error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e) + (log[0] ? ": " : "") + log);
-    capture_phase("instantiated", nnodes);
+    HIPCHECK(hipEventCreateWithFlags(&cb->done, hipEventDisableTiming));
+    HIPCHECK(hipGraphCreate(&cb->graph, 0));
+    HipGraphBuilder gb(cb->graph);
+    build_jobs_graph(gb, jobs, temps);
+    HIPCHECK(hipGraphInstantiate(&cb->exec, cb->graph, nullptr, nullptr, 0));
     return cb.release();
 }
 
+void launch_captured(CapturedBatch &cb, hipStream_t st) {
+    HIPCHECK(hipSetDevice(cb.device));
+    if (!st) st = cb.ex->stream;
+    // replays on other streams wait for the previous replay (shared temporaries)
+    if (cb.armed) HIPCHECK(hipStreamWaitEvent(st, cb.done, 0));
+    HIPCHECK(hipGraphLaunch(cb.exec, st));
+    HIPCHECK(hipEventRecord(cb.done, st));
+    cb.armed = true;
+}
+
 CapturedBatch::~CapturedBatch() {
+    if (done) hipEventSynchronize(done);
     if (exec) hipGraphExecDestroy(exec);
     if (graph) hipGraphDestroy(graph);
+    if (done) hipEventDestroy(done);
     for (auto &t : temps) free_device_temps(t);
+}
+
+// Host-only description of the graph capture_jobs would build (no device
+// calls): one line per node, "node job stage kernel grid block deps...".
+namespace {
+struct RecordingBuilder : GraphBuilder {
+    std::string text;
+    int count = 0, job = 0, stage = 0, kern = 0;
+    std::vector<std::unique_ptr<int>> ids;
+    void tag(int j, int s, int k) override {
+        job = j;
+        stage = s;
+        kern = k;
+    }
+    void *record(const char *what, dim3 grid, dim3 block, const std::vector<void *> &deps) {
+        ids.emplace_back(new int(count++));
+        void *h = ids.back().get();
+        text += std::to_string(*(int *)h) + " " + what + " " + std::to_string(job) + " " + std::to_string(stage) +
+                " " + std::to_string(kern) + " " + std::to_string(grid.x) + " " + std::to_string(block.x) + " :";
+        for (void *d : deps) text += " " + std::to_string(*(int *)d);
+        text += "\n";
+        return h;
+    }
+    hipError_t add_kernel(const void *, dim3 grid, dim3 block, void **, const std::vector<void *> &deps,
+                          void **node) override {
+        *node = record("kernel", grid, block, deps);
+        return hipSuccess;
+    }
+    hipError_t add_empty(const std::vector<void *> &deps, void **node) override {
+        *node = record("empty", dim3(0), dim3(0), deps);
+        return hipSuccess;
+    }
+};
+} // namespace
+
+std::string describe_jobs_graph(const std::vector<BatchJob> &jobs, int lanes) {
+    check_jobs(jobs);
+    std::vector<DeviceTemps> own(jobs.size());
+    std::vector<const DeviceTemps *> temps;
+    for (size_t j = 0; j < jobs.size(); j++) {
+        own[j].lay = batch_layout(jobs[j].m, jobs[j].S, jobs[j].L, ChunkPolicy::from_env(), lanes);
+        // a placeholder arena base (never dereferenced): the launches see a
+        // non-null arena as on the device
+        own[j].arena = (char *)(uintptr_t)((j + 1) << 20);
+        own[j].bufs.assign(jobs[j].m->nbuf, nullptr);
+        for (int b = 0; b < jobs[j].m->nbuf; b++)
+            if (jobs[j].m->bufs[b].kind == KUN_BUF_TEMP) own[j].bufs[b] = own[j].arena + own[j].lay.buf_off[b];
+        temps.push_back(&own[j]);
+    }
+    RecordingBuilder rb;
+    build_jobs_graph(rb, jobs, temps);
+    for (auto &t : own) t.arena = nullptr; // not memory of ours
+    // the launch DAGs the graph follows: "dag job node stage kernel : deps"
+    for (size_t j = 0; j < jobs.size(); j++)
+        for (size_t n = 0; n < own[j].lay.nodes.size(); n++) {
+            const auto &nd = own[j].lay.nodes[n];
+            rb.text += "dag " + std::to_string(j) + " " + std::to_string(n) + " " + std::to_string(nd.stage) + " " +
+                       std::to_string(nd.kern) + " :";
+            for (int d : nd.deps) rb.text += " " + std::to_string(d);
+            rb.text += "\n";
+        }
+    return rb.text;
 }
 
 void Executor::free_temps() {

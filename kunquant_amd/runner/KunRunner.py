@@ -71,6 +71,8 @@ def _load():
     sig("kunCaptureGraphBatch", vp, vp, i, ctypes.POINTER(vp), ctypes.POINTER(vp), szp, szp, szp, szp)
     sig("kunLaunchGraphBatch", i, vp, vp)
     sig("kunDestroyGraphBatch", None, vp)
+    sig("kunDescribeGraphBatch", i, i, ctypes.POINTER(vp), szp, szp, szp, szp, i, ctypes.c_char_p, szp)
+    sig("kunModuleLoadSerial", ctypes.c_ulonglong, vp)
     sig("kunModuleStageNumKernels", i, vp, i)
     fpp = ctypes.POINTER(vp)
     sig("kunCorrWith", i, vp, i, i, i, fpp, vp, fpp, sz, sz, sz, sz)
@@ -532,9 +534,28 @@ def runGraphBatchDevice(exec: Executor, jobs, stream: int = 0):
         raise RuntimeError(_err())
 
 
+def describeGraphBatch(jobs, lanes: int = 6) -> str:
+    """Host-only text description of the HIP graph BatchGraph builds for
+    jobs [(module, num_stocks, total_time, cur_time, length)]: one line per
+    graph node ("<id> kernel|empty <job> <stage> <kernel> <grid> <block> :
+    <deps>") and per launch-DAG node ("dag <job> <node> <stage> <kernel> :
+    <deps>")."""
+    L = _load()
+    n = len(jobs)
+    mods = (ctypes.c_void_p * n)(*[j[0]._h for j in jobs])
+    cols = [(ctypes.c_size_t * n)(*[int(j[k]) for j in jobs]) for k in range(1, 5)]
+    size = ctypes.c_size_t(0)
+    if L.kunDescribeGraphBatch(n, mods, *cols, lanes, None, ctypes.byref(size)):
+        raise RuntimeError(_err())
+    buf = ctypes.create_string_buffer(size.value)
+    if L.kunDescribeGraphBatch(n, mods, *cols, lanes, buf, ctypes.byref(size)):
+        raise RuntimeError(_err())
+    return buf.value.decode()
+
+
 class BatchGraph:
-    """A batch of jobs captured once as a HIP graph and replayed with one
-    launch (device pointers are fixed at capture)."""
+    """A batch of jobs built once as a HIP graph and replayed with one
+    launch (device pointers are fixed when it is built)."""
 
     def __init__(self, exec: Executor, jobs):
         L = _load()

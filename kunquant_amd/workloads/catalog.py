@@ -6,7 +6,7 @@ import os
 from typing import Callable, Dict, Tuple
 
 from .. import PACKAGE_DIR
-from .graphs import (aligned_graph, alpha158_graph, cross_section_graph, generic_cs_graph, tsrank_chain_graph, ops_graph, precision_graph, random_graph, simple_graph,
+from .graphs import (aligned_graph, capi_batch_graph, capi_stream_graph, alpha158_graph, cross_section_graph, generic_cs_graph, tsrank_chain_graph, ops_graph, precision_graph, random_graph, simple_graph,
                      stream_graph)
 
 MODULE_DIR = os.path.join(PACKAGE_DIR, "lib", "modules")
@@ -25,6 +25,10 @@ CATALOG: Dict[str, Tuple[Callable, dict]] = {
                                                         options={"no_fast_stat": "no_warn"})),
     "streamgraph_batch_f64": (stream_graph, dict(dtype="double", input_layout="TS", output_layout="TS",
                                                   options={"no_fast_stat": True})),
+    # the modules of the reference's C API test (names included)
+    "testRuntimeModule": (capi_batch_graph, dict(input_layout="STs", output_layout="STs")),
+    "alpha_101_stream": (capi_stream_graph, dict(partition_factor=8, input_layout="STREAM", output_layout="STREAM",
+                                                 options={"opt_reduce": False, "fast_log": True})),
     "aligned_f32": (aligned_graph, dict(input_layout="TS", output_layout="TS", allow_unaligned=False)),
     "cs_f32": (cross_section_graph, dict(input_layout="TS", output_layout="TS")),
     # the Alpha158 factor library as the reference's test compiles it (fast

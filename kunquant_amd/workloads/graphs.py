@@ -238,6 +238,33 @@ def stream_graph(K=None):
     return K.Function(b.ops)
 
 
+def capi_batch_graph(K=None):
+    """output = input + input * 2 over one input (the module the reference's
+    C API test runs in STs layout)."""
+    K = K or native()
+    b = K.Builder()
+    with b:
+        inp = K.Input("input")
+        K.Output(inp + inp * 2, "output")
+    return K.Function(b.ops)
+
+
+def capi_stream_graph(K=None):
+    """alpha101 = (close - open) / ((high - low) + 0.001) beside windowed
+    factors over the six fields, so that a stream's serialised state holds
+    rings and op state (the stream module of the reference's C API test)."""
+    K = K or native()
+    b = K.Builder()
+    with b:
+        close, opn, high, low, vol, amount = (K.Input(n) for n in FIELDS[3:4] + FIELDS[0:1] + FIELDS[1:3]
+                                              + FIELDS[4:6])
+        K.Output((close - opn) / ((high - low) + 0.001), "alpha101")
+        K.Output(K.WindowedCorrelation(close, 10, vol), "corr10")
+        K.Output(K.ExpMovingAvg(amount, 5), "ema5")
+        K.Output(K.Rank(K.WindowedSum(high - low, 5)), "rank_range5")
+    return K.Function(b.ops)
+
+
 def aligned_graph(K=None):
     """Rank(a) * 2; compiled with allow_unaligned=False."""
     K = K or native()

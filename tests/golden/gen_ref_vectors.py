@@ -265,7 +265,10 @@ def main():
         # reference builds of bench graphs with no fixture: bench.py times
         # them as its CPU baseline and compares their outputs with the
         # device's (oracle/_ref only, nothing committed)
+        by_case = {c[0]: c for c in CASES}
         for name, factory, cfg in [
+                ("rand_f32", by_case["rand_f32"][1], by_case["rand_f32"][2]),
+                ("alpha158_f64", by_case["alpha158_f64"][1], by_case["alpha158_f64"][2]),
                 ("cfg2_f64", lambda K: wgraphs.random_graph(100, 102, K=K),
                  dict(dtype="double", input_layout="TS", output_layout="TS", options={"no_fast_stat": "no_warn"})),
                 ("prec_f32", lambda K: wgraphs.precision_graph(150, 3, K=K),

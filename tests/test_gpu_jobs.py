@@ -89,12 +89,11 @@ def test_batched_jobs_match_alone(batch):
     _check(_fetch(items), alone, "batched (second call)")
 
 
-@pytest.mark.skip(reason="batch graph capture is switched off: kunCaptureGraphBatch ended the process with a "
-                         "segmentation fault on MI355X, cause not found yet")
 def test_batch_graph_replays_match_alone(batch):
-    """The captured batch replayed three times, with eager device work (a
-    plain device run of one job into other buffers, torch fills of every
-    output) between the replays; every replay is checked."""
+    """The batch as one HIP graph (built node by node from the launch DAGs),
+    replayed three times, with eager device work (a plain device run of one
+    job into other buffers, torch fills of every output) between the
+    replays; every replay is checked."""
     ex, items, alone = batch
     g = kr.BatchGraph(ex, _job_tuples(items))
     sp0, mod0, _, di0, do0 = items[0]

@@ -151,11 +151,16 @@ def _cs_rows(S, dt, seed):
         x = rng.standard_normal(S)
         x[rng.random(S) < 0.1] = np.nan
         rows.append(x)
+    # magnitudes over many binades (every key byte differs across the row)
+    # and near ties one unit in the last place apart (only the low bytes do)
+    rows.append(rng.standard_normal(S) * 10.0 ** rng.uniform(-30, 30, S))
+    one = np.asarray(1.0, dt)
+    rows.append(one + rng.integers(0, 5, S) * np.finfo(dt).eps)
     return np.ascontiguousarray(np.stack(rows).astype(dt))
 
 
 @pytest.mark.parametrize("module", ["cs_f32", "cs_f64"])
-@pytest.mark.parametrize("S", [1, 5, 64, 257, 1000, 8192, 16384])
+@pytest.mark.parametrize("S", [1, 5, 64, 257, 1000, 2000, 4000, 8192, 16384])
 def test_cross_section_edges(executor, module, S):
     """Rank / Scale rows: all NaN, all zero, heavy ties, signed zeros,
     infinities, a single valid value, random rows with NaN cells."""

@@ -77,3 +77,38 @@ def test_module_errors():
     with pytest.raises(RuntimeError, match="Cannot load library"):
         kr.Library.load("/nonexistent/lib.so")
     assert mod.getOutputUnreliableCount() == {"x": 0, "std20": 20, "vsum10": 10, "rank_x": 0}
+
+
+def test_device_runs_on_two_streams_share_temporaries_safely(executor):
+    """The same module and shape run back to back on two different torch
+    streams: both runs use the executor's cached temporaries, so the second
+    must wait for the first (whatever the streams) -- both results equal the
+    oracle bit for bit. The first stream is held busy by a long torch kernel
+    so that, without the wait, the second run would overtake it."""
+    import torch
+    from kunquant_amd.codegen.program import program_text
+    from oracle.kun_oracle import OracleProgram
+    _, mod = catalog.load("cfg2_f32")
+    S, T = 640, 160
+    runs = []
+    for seed in (11, 12):
+        d = data.ohlcv(S, T, seed=seed, nan_frac=0.1)
+        dev_in = {k: torch.from_numpy(v).cuda() for k, v in d.items()}
+        dev_out = {k: torch.full((T, S), 7.0, dtype=torch.float32, device="cuda") for k in mod.getOutputNames()}
+        runs.append((d, dev_in, dev_out))
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    big = torch.rand(4096, 4096, device="cuda")
+    streams[0].wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(streams[0]):
+        for _ in range(8):
+            big = big @ big * 1e-3   # keep stream 0 busy before its graph run
+    for (d, dev_in, dev_out), st in zip(runs, streams):
+        ptrs = {k: v.data_ptr() for k, v in list(dev_in.items()) + list(dev_out.items())}
+        kr.runGraphDevice(executor, mod, ptrs, S, T, 0, T, stream=st.cuda_stream)
+    torch.cuda.synchronize()
+    prog = OracleProgram(program_text(catalog.module_ir("cfg2_f32")))
+    for i, (d, _, dev_out) in enumerate(runs):
+        exp = prog.run({k: d[k] for k in mod.getInputNames()}, S, T, 0, T, nthreads=16)
+        bad = [k for k in exp if not np.array_equal(dev_out[k].cpu().numpy(), exp[k], equal_nan=True)]
+        assert not bad, f"run {i} on its own stream: outputs {bad[:3]} differ from the oracle"

@@ -172,6 +172,21 @@ KUN_API void kunDestroyGraphBatch(KunBatchGraphHandle g);
 KUN_API int kunDescribeGraphBatch(int njobs, const KunModuleHandle *mods, const size_t *num_stocks,
                                   const size_t *total_time, const size_t *cur_time, const size_t *length, int lanes,
                                   char *buffer, size_t *size);
+/* Bytes of the fixed device arena of kunRunGraphDeviceNoAlloc (per device). */
+#define KUN_NOALLOC_SCRATCH_BYTES (64u << 20)
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc). kunRunGraphDevice without any allocation: the run's
+ * temporaries, stage-kernel scratch and row sums live in one fixed arena of
+ * KUN_NOALLOC_SCRATCH_BYTES per device, held by the library from the first
+ * call on a device until the process ends. A run whose memory plan needs
+ * more fails with an error (kunGetLastError) and enqueues nothing. Kernels
+ * go in launch order on `hip_stream` (NULL: the executor's stream); returns
+ * without synchronising. Every call on a device shares that arena: calls
+ * that overlap on one device race (order them on one stream). */
+KUN_API int kunRunGraphDeviceNoAlloc(KunExecutorHandle exec, KunModuleHandle m, KunBufferNameMapHandle buffers,
+                                     size_t num_stocks, size_t total_time, size_t cur_time, size_t length,
+                                     void *hip_stream);
 /* Load serial of the library the module handle was obtained from: every
  * successful kunLoadLibrary gets a new one. The executor's caches (device
  * temporaries, chunking, stage order, timings) key by (module, serial), so a

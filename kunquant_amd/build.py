@@ -15,7 +15,7 @@ MODULE_DIR = os.path.join(LIB_DIR, "modules")
 ARCH = os.environ.get("KUN_OFFLOAD_ARCH", "gfx950")
 
 RUNTIME_SOURCES = ["runtime.hip", "stream.hip", "stream_pool.hip", "cross_section.hip", "row_reduce.hip",
-                   "capi.cpp"]
+                   "noalloc.hip", "capi.cpp"]
 
 
 def hipcc() -> str:

@@ -6,7 +6,7 @@ import os
 from typing import Callable, Dict, Tuple
 
 from .. import PACKAGE_DIR
-from .graphs import (aligned_graph, capi_batch_graph, capi_stream_graph, alpha158_graph, cross_section_graph, generic_cs_graph, tsrank_chain_graph, ops_graph, precision_graph, random_graph, simple_graph,
+from .graphs import (aligned_graph, alpha101_graph, large_window_graph, capi_batch_graph, capi_stream_graph, alpha158_graph, cross_section_graph, generic_cs_graph, tsrank_chain_graph, ops_graph, precision_graph, random_graph, simple_graph,
                      stream_graph)
 
 MODULE_DIR = os.path.join(PACKAGE_DIR, "lib", "modules")
@@ -36,6 +36,15 @@ CATALOG: Dict[str, Tuple[Callable, dict]] = {
     "alpha158_f64": (alpha158_graph, dict(dtype="double", blocking_len=4, partition_factor=4, input_layout="TS",
                                           output_layout="TS", options={"opt_reduce": True, "fast_log": True,
                                                                        "no_fast_stat": "no_warn"})),
+    # the Alpha101 library compiled as the reference's test compiles it
+    # (fp32: fast log, rolling sums; fp64: fast statistics on as well)
+    "alpha101_f32": (alpha101_graph, dict(input_layout="TS", output_layout="TS",
+                                          options={"opt_reduce": True, "fast_log": True})),
+    "alpha101_f64": (alpha101_graph, dict(dtype="double", input_layout="TS", output_layout="TS",
+                                          options={"opt_reduce": True, "fast_log": True, "no_fast_stat": "no_warn"})),
+    # 200-row windows: skip-list TsRank / Min / Max / ArgMin / ArgMax (fp64)
+    "skiplist_f64": (large_window_graph, dict(dtype="double", input_layout="TS", output_layout="TS",
+                                              options={"no_fast_stat": "no_warn"})),
     # rank of a rank (the reference's copy-constructed rank state)
     "tsrank_chain_f32": (tsrank_chain_graph, dict(input_layout="TS", output_layout="TS")),
     # generic cross-sectional op (device-compiled body) in TS, STs and stream modules

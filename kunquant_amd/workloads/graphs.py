@@ -301,6 +301,37 @@ def alpha158_graph(K=None, AllData=None):
     return K.Function(b.ops)
 
 
+def large_window_graph(K=None):
+    """TsRank, WindowedMin/Max, TsArgMin/TsArgMax over 200 rows: windows
+    long enough for the reference's skip-list form in fp64
+    (CompOp.py:14-17)."""
+    K = K or native()
+    b = K.Builder()
+    with b:
+        a = K.Input("a")
+        K.Output(K.TsRank(a, 200), "rank200")
+        K.Output(K.WindowedMin(a, 200), "min200")
+        K.Output(K.WindowedMax(a, 200), "max200")
+        K.Output(K.TsArgMin(a, 200), "argmin200")
+        K.Output(K.TsArgMax(a, 200), "argmax200")
+    return K.Function(b.ops)
+
+
+def alpha101_graph(K=None, library=None):
+    """Every alpha of an Alpha101 library (`library`: a module with
+    AllData and all_alpha; default this package's) over the six fields."""
+    K = K or native()
+    if library is None:
+        from ..predefined import Alpha101 as library
+    b = K.Builder()
+    with b:
+        d = library.AllData(low=K.Input("low"), high=K.Input("high"), close=K.Input("close"), open=K.Input("open"),
+                            amount=K.Input("amount"), volume=K.Input("volume"))
+        for f in library.all_alpha:
+            K.Output(f(d), f.__name__)
+    return K.Function(b.ops)
+
+
 def tsrank_chain_graph(K=None):
     """A factor of the 100-factor graph (f022) with its intermediates as
     outputs: TsArgMax over TsRank over TsRank over a rolling sum of the

@@ -221,6 +221,34 @@ def alpha158_ref_graph(K):
 CHAIN_STOCKS = [128, 4774, 4775]
 
 
+def alpha101_ref_graph(K):
+    sys.path.insert(0, REF)
+    import KunQuant.predefined.Alpha101 as lib
+    return wgraphs.alpha101_graph(K, library=lib)
+
+
+def alpha101_inputs(dtype, seed):
+    """Prices and volumes with the share of flat moves real data has (equal
+    closes, high == low days) so that the 0.0001 divisor guards and ties
+    are exercised."""
+    d = wdata.ohlcv(20, 320, seed=seed, dtype=dtype, nan_frac=0.0)
+    rng = np.random.Generator(np.random.PCG64(seed + 1))
+    flat = rng.random((320, 20)) < 0.03
+    for k in ("open", "high", "low"):
+        d[k] = np.where(flat, d["close"], d[k]).astype(dtype)
+    return d
+
+
+def large_window_inputs(S, T, seed):
+    """Uniform values with a block of repeated minima and NaN cells (ties
+    and gaps inside 200-row windows)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    a = rng.random((T, S))
+    a[400:410, :] = -1
+    a[rng.random((T, S)) < 0.002] = np.nan
+    return {"a": a}
+
+
 def generic_inputs(S, T, dtype, seed):
     rng = np.random.Generator(np.random.PCG64(seed))
     d = {k: rng.random((T, S)).astype(dtype) for k in ("a", "b", "c")}
@@ -235,6 +263,18 @@ CASES = [
      dict(dtype="double", blocking_len=4, partition_factor=4, input_layout="TS", output_layout="TS",
           options={"opt_reduce": True, "fast_log": True}),
      np.float64, lambda: wdata.ohlcv(16, 200, seed=158, dtype=np.float64, nan_frac=0.02), 16, 200, 0, 200),
+    # the reference's own Alpha101 library, compiled as its test_alpha101.py:27-47 does
+    ("alpha101_f32", alpha101_ref_graph, dict(input_layout="TS", output_layout="TS",
+                                              options={"opt_reduce": True, "fast_log": True}),
+     np.float32, lambda: alpha101_inputs(np.float32, 101), 20, 320, 0, 320, 256),
+    ("alpha101_f64", alpha101_ref_graph, dict(dtype="double", input_layout="TS", output_layout="TS",
+                                              options={"opt_reduce": True, "fast_log": True,
+                                                       "no_fast_stat": "no_warn"}),
+     np.float64, lambda: alpha101_inputs(np.float64, 102), 20, 320, 0, 320, 256),
+    # 200-row windows: the reference's skip-list forms (fp64, test_runtime.py:271-300's shape)
+    ("skiplist_f64", wgraphs.large_window_graph, dict(dtype="double", input_layout="TS", output_layout="TS",
+                                                      options={"no_fast_stat": "no_warn"}),
+     np.float64, lambda: large_window_inputs(24, 1200, 44), 24, 1200, 0, 1200, 900),
     ("generic_f32", wgraphs.generic_cs_graph, dict(input_layout="TS", output_layout="TS"), np.float32,
      lambda: generic_inputs(40, 60, np.float32, 21), 40, 60, 0, 60),
     ("simple_f32", wgraphs.simple_graph, dict(input_layout="TS", output_layout="TS"), np.float32,
@@ -281,15 +321,18 @@ def main():
     # --stream-only: rewrite the stream fixtures alone; --only=<case>: one batch case
     only = [a.split("=", 1)[1] for a in sys.argv if a.startswith("--only=")]
     batch_cases = [] if "--stream-only" in sys.argv else [c for c in CASES if not only or c[0] in only]
-    for name, factory, cfg, dtype, make_inputs, S, T, start, L in batch_cases:
+    for name, factory, cfg, dtype, make_inputs, S, T, start, L, *keep in batch_cases:
         f = factory(K)
         outs = output_names(f)
         so = ref_compile(K, f, name, dict(cfg))
         inputs = make_inputs()
         res = run_batch(so, name, inputs, outs, S, T, start, L, dtype)
+        # large libraries: only output rows keep[0].. are stored (meta[4])
+        k0 = keep[0] if keep else 0
         np.savez_compressed(os.path.join(HERE, f"ref_{name}.npz"),
-                            **{f"in_{k}": v for k, v in inputs.items()}, **{f"out_{k}": v for k, v in res.items()},
-                            meta=np.array([S, T, start, L]))
+                            **{f"in_{k}": v for k, v in inputs.items()},
+                            **{f"out_{k}": np.ascontiguousarray(v[k0:]) for k, v in res.items()},
+                            meta=np.array([S, T, start, L] + ([k0] if keep else [])))
         print("wrote", name, len(outs), "outputs")
     if only and "tsrank_chain" in only:
         # the bench's volume (8192 stocks x 1024 days, regenerated from its

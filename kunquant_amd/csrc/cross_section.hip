@@ -65,16 +65,22 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
 }
 
 // lanes of the wavefront holding the same 8-bit digit as this lane: one
-// ballot per digit bit, each lane keeping the lanes that agree on that bit
-__device__ __forceinline__ uint64_t same_digit(uint32_t d) {
-    uint64_t peers = ~0ull;
+// ballot per digit bit; a lane disagrees with the lanes whose ballot bit
+// differs from its own bit (m: all ones when this lane's bit is set), and
+// the peers are the lanes that never disagree
+// (bits on which the row's keys differ, `vary`, uniform: the others tell no
+// lanes apart and are skipped)
+__device__ __forceinline__ uint64_t same_digit(uint32_t d, uint32_t vary) {
+    uint32_t miss_lo = 0, miss_hi = 0;
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
-        const bool set = (d >> b) & 1u;
-        const uint64_t bal = __ballot(set);
-        peers &= set ? bal : ~bal;
+        if (!((vary >> b) & 1u)) continue;
+        const uint32_t m = (uint32_t)((int32_t)(d << (31 - b)) >> 31);
+        const uint64_t bal = __ballot(m != 0u);
+        miss_lo |= (uint32_t)bal ^ m;
+        miss_hi |= (uint32_t)(bal >> 32) ^ m;
     }
-    return peers;
+    return ~(((uint64_t)miss_hi << 32) | miss_lo);
 }
 
 template <typename V> __device__ __forceinline__ V wave_incl_sum(V v) {
@@ -175,26 +181,36 @@ template <typename T, int BLOCK, int ITEMS, bool KEYS_ONLY> struct RadixRank {
 
     static __device__ __forceinline__ void row(Smem &sm, const kun_cs::View &in, const kun_cs::View &out, long S,
                                                int blk, long t) {
+        if (S <= 0) return; // uniform: nothing to rank
         const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
         const int base = w * 64 * ITEMS + lane;
         const T *src = (const T *)in.ptr;
         T *dst = (T *)out.ptr;
         const long rin = t + in.row0;
         K key[ITEMS];
-        int id[KEYS_ONLY ? 1 : ITEMS];
+        // stock of each key, two 16-bit ids per register
+        uint32_t id2[KEYS_ONLY ? 1 : (ITEMS + 1) / 2];
         int cnt = 0;
         K kand = ~K(0), kor = K(0);
+        // every load issued before any is used (stocks past S read the last
+        // stock's cell and are then replaced by padding keys)
+        T xs[ITEMS];
 #pragma unroll
         for (int j = 0; j < ITEMS; j++) {
             const long s = base + j * 64;
-            K k = ~K(0);
-            if (s < S) {
-                const T x = src[cs_at(in.layout, rin, s, S, in.rows, blk)];
-                k = rank_key(x);
-                cnt += (x == x) ? 1 : 0;
-            }
+            xs[j] = src[cs_at(in.layout, rin, s < S ? s : S - 1, S, in.rows, blk)];
+        }
+#pragma unroll
+        for (int j = 0; j < ITEMS; j++) {
+            const long s = base + j * 64;
+            const bool live = s < S;
+            const K k = live ? rank_key(xs[j]) : ~K(0);
+            cnt += (live && xs[j] == xs[j]) ? 1 : 0;
             key[j] = k;
-            if constexpr (!KEYS_ONLY) id[j] = (int)s;
+            if constexpr (!KEYS_ONLY) {
+                if (j % 2 == 0) id2[j / 2] = (uint32_t)s;
+                else id2[j / 2] |= (uint32_t)s << 16;
+            }
             kand &= k;
             kor |= k;
         }
@@ -225,30 +241,42 @@ template <typename T, int BLOCK, int ITEMS, bool KEYS_ONLY> struct RadixRank {
             // this wavefront's row only: its reads of the row in the
             // previous pass ended before the last barrier
             for (int q = lane; q < 256; q += 64) hw[q] = 0;
-            uint32_t loc[ITEMS];
+            // rank of each key within its digit among this wavefront's keys
+            // before it, two 16-bit ranks per register
+            uint32_t loc2[(ITEMS + 1) / 2];
+            const uint32_t vary = (uint32_t)(diff >> sh) & 0xffu;
 #pragma unroll
             for (int j = 0; j < ITEMS; j++) {
                 const uint32_t d = (uint32_t)(key[j] >> sh) & 0xffu;
-                const uint64_t peers = same_digit(d);
+                const uint64_t peers = same_digit(d, vary);
                 const uint32_t below = lanes_below(peers);
                 const uint32_t prior = hw[d];
                 if (below == 0) hw[d] = prior + (uint32_t)__popcll(peers);
-                loc[j] = prior + below;
+                if (j % 2 == 0) loc2[j / 2] = prior + below;
+                else loc2[j / 2] |= (prior + below) << 16;
+                // one key at a time: interleaving the keys' ballot chains
+                // buys no speed and costs the registers of a second
+                // resident workgroup
+                __builtin_amdgcn_sched_barrier(0);
             }
             __syncthreads();
             scan_hist(sm);
             __syncthreads();
 #pragma unroll
             for (int j = 0; j < ITEMS; j++) {
-                const uint32_t pos = hw[(uint32_t)(key[j] >> sh) & 0xffu] + loc[j];
+                const uint32_t pos = hw[(uint32_t)(key[j] >> sh) & 0xffu] + ((loc2[j / 2] >> (16 * (j % 2))) & 0xffffu);
                 sm.key[pos] = key[j];
-                if constexpr (!KEYS_ONLY) sm.idx[pos] = (uint16_t)id[j];
+                if constexpr (!KEYS_ONLY) sm.idx[pos] = (uint16_t)(id2[j / 2] >> (16 * (j % 2)));
             }
             __syncthreads();
 #pragma unroll
             for (int j = 0; j < ITEMS; j++) {
                 key[j] = sm.key[base + j * 64];
-                if constexpr (!KEYS_ONLY) id[j] = sm.idx[base + j * 64];
+                if constexpr (!KEYS_ONLY) {
+                    const uint32_t v = sm.idx[base + j * 64];
+                    if (j % 2 == 0) id2[j / 2] = v;
+                    else id2[j / 2] |= v << 16;
+                }
             }
         }
         if (!moved) {
@@ -262,22 +290,28 @@ template <typename T, int BLOCK, int ITEMS, bool KEYS_ONLY> struct RadixRank {
 #pragma unroll
             for (int j = 0; j < ITEMS; j++) {
                 const long s = base + j * 64;
-                key[j] = s < S ? rank_key(src[cs_at(in.layout, rin, s, S, in.rows, blk)]) : ~K(0);
+                key[j] = rank_key(src[cs_at(in.layout, rin, s < S ? s : S - 1, S, in.rows, blk)]);
             }
+#pragma unroll
+            for (int j = 0; j < ITEMS; j++)
+                if (base + j * 64 >= S) key[j] = ~K(0);
             rank_by_search(sm, key, n, dst, out, S, blk, t, base);
         } else {
-            rank_by_groups(sm, key, id, n, dst, out, S, blk, t, base);
+            rank_by_groups(sm, key, id2, n, dst, out, S, blk, t, base);
         }
     }
 
     static __device__ __forceinline__ T rank_of(int a, int b, int n) {
-        const int64_t sum = ((int64_t)a + b + 1) * (int64_t)(b - a) / 2;
+        // rows of at most 16384 stocks: (a + b + 1) * (b - a) < 2^31, so the
+        // reference's integer sum fits 32 bits and converts to T the same way
+        const int32_t sum = (a + b + 1) * (b - a) / 2;
         // x / 1 == x exactly: a group of one skips the first division
         const T q = (b - a == 1) ? (T)sum : (T)sum / (T)(b - a);
         return q / (T)n;
     }
 
-    static __device__ __forceinline__ void rank_by_groups(Smem &sm, const K (&key)[ITEMS], const int (&id)[ITEMS],
+    static __device__ __forceinline__ void rank_by_groups(Smem &sm, const K (&key)[ITEMS],
+                                                          const uint32_t (&id2)[(ITEMS + 1) / 2],
                                                           int n, T *dst, const kun_cs::View &out, long S, int blk,
                                                           long t, int base) {
         const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -317,8 +351,10 @@ template <typename T, int BLOCK, int ITEMS, bool KEYS_ONLY> struct RadixRank {
         __syncthreads();
         T *res = (T *)sm.key;
 #pragma unroll
-        for (int j = 0; j < ITEMS; j++)
-            if (id[j] < S) res[id[j]] = r[j];
+        for (int j = 0; j < ITEMS; j++) {
+            const int id = (int)((id2[j / 2] >> (16 * (j % 2))) & 0xffffu);
+            if (id < S) res[id] = r[j];
+        }
         __syncthreads();
         for (long s = threadIdx.x; s < S; s += BLOCK) dst[cs_at(out.layout, t + out.row0, s, S, out.rows, blk)] = res[s];
     }

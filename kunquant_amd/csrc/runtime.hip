@@ -842,9 +842,13 @@ void run_jobs_device(Executor &ex, const std::vector<BatchJob> &jobs, hipStream_
 // ---------------------------------------------------------------------------
 // A batch of jobs as one HIP graph, built node by node (hipGraphAddKernelNode)
 // from the jobs' launch DAGs rather than captured from streams: no events,
-// no lanes, no stream capture. Each DAG node contributes its kernels, the
-// first depending on the last kernel of each of the node's dependencies,
-// later ones on the kernel before them; jobs are independent subgraphs.
+// no stream capture. Each DAG node contributes its kernels, the first
+// depending on the last kernel of each of the node's dependencies and of the
+// node before it on the same lane, later ones on the kernel before them;
+// jobs are independent subgraphs. The lane edges matter: the arena's memory
+// plan (plan_slots) reuses a slot once its previous users precede the new
+// writer through DAG edges or through the order of a lane, as on the
+// executor's streams.
 
 namespace {
 struct GraphSink : kun_cs::Sink {
@@ -897,10 +901,16 @@ void build_jobs_graph(GraphBuilder &gb, const std::vector<BatchJob> &jobs, const
         const DeviceTemps &t = *temps[j];
         const std::vector<void *> dev = device_pointers(jb.m, *jb.bufs, t);
         std::vector<void *> out(t.lay.nodes.size(), nullptr);
+        std::map<int, int> last_on_lane;
         for (size_t n = 0; n < t.lay.nodes.size(); n++) {
             const auto &nd = t.lay.nodes[n];
             GraphSink sink(gb);
-            for (int d : nd.deps) {
+            std::vector<int> pred = nd.deps;
+            auto ln = last_on_lane.find(nd.lane);
+            if (ln != last_on_lane.end() && std::find(pred.begin(), pred.end(), ln->second) == pred.end())
+                pred.push_back(ln->second);
+            last_on_lane[nd.lane] = (int)n;
+            for (int d : pred) {
                 if (d < 0 || (size_t)d >= n || !out[d]) throw Error("launch DAG out of topological order");
                 sink.deps.push_back(out[d]);
             }
@@ -1006,12 +1016,12 @@ std::string describe_jobs_graph(const std::vector<BatchJob> &jobs, int lanes) {
     RecordingBuilder rb;
     build_jobs_graph(rb, jobs, temps);
     for (auto &t : own) t.arena = nullptr; // not memory of ours
-    // the launch DAGs the graph follows: "dag job node stage kernel : deps"
+    // the launch DAGs the graph follows: "dag job node stage kernel lane : deps"
     for (size_t j = 0; j < jobs.size(); j++)
         for (size_t n = 0; n < own[j].lay.nodes.size(); n++) {
             const auto &nd = own[j].lay.nodes[n];
             rb.text += "dag " + std::to_string(j) + " " + std::to_string(n) + " " + std::to_string(nd.stage) + " " +
-                       std::to_string(nd.kern) + " :";
+                       std::to_string(nd.kern) + " " + std::to_string(nd.lane) + " :";
             for (int d : nd.deps) rb.text += " " + std::to_string(d);
             rb.text += "\n";
         }

@@ -297,9 +297,77 @@ STREAM_CASES = [
 ]
 
 
+AGG_KINDS = ("sum", "min", "max", "first", "last", "count", "mean")
+
+
+def rowreduce_vectors():
+    """corrWith (TS and STs, plain and ranked inputs) and aggregrate (float,
+    double) through the reference runtime's own functions (a harness built
+    from tests/golden/ref_rowreduce.cpp): tests/golden/ref_rowreduce.npz."""
+    so = os.path.join(REF_OUT, "ref_rowreduce.so")
+    subprocess.check_call(["g++", "-std=c++11", "-O2", "-fPIC", "-shared"] + CONTRACT +
+                          ["-I", os.path.join(REF, "cpp"), os.path.join(HERE, "ref_rowreduce.cpp"), "-o", so,
+                           "-L", REF_OUT, "-lKunRuntime", f"-Wl,-rpath,{REF_OUT}"])
+    capi()  # loads libKunRuntime.so globally
+    H = ctypes.CDLL(so)
+    fp = ctypes.POINTER(ctypes.c_void_p)
+    vp, sz, i = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int
+    H.ref_corr_with.argtypes = [i, i, i, fp, vp, fp, sz, sz, sz, sz]
+    H.ref_aggregrate.argtypes = [i, fp, fp, i, fp, sz, sz, sz, sz]
+    ptrs = lambda arrs: (vp * len(arrs))(*[None if a is None else a.ctypes.data for a in arrs])
+    rng = np.random.Generator(np.random.PCG64(77))
+    out = {}
+    # corrWith: S = 37 (not a multiple of 8 in TS), 60 rows, one all-NaN row
+    # and NaN cells (plain: NaN rows out; ranked: NaN cells drop out of the
+    # ranks), ties for the ranked path
+    S, T = 37, 60
+    a = rng.random((T, S)).astype(np.float32)
+    b = np.floor(rng.random((T, S)) * 6).astype(np.float32)
+    r = rng.random((T, S)).astype(np.float32)
+    a[7, 3] = np.nan
+    b[11, :] = np.nan
+    out.update(cw_a=a, cw_b=b, cw_r=r)
+    for rank in (0, 1):
+        o = [np.full(T, -7, np.float32) for _ in range(2)]
+        H.ref_corr_with(0, rank, 2, ptrs([a, b]), r.ctypes.data, ptrs(o), S, T, 0, T)
+        out[f"cw_ts_rank{rank}"] = np.stack(o)
+    # STs layout: 40 stocks (blocks of 8), rows 5..59
+    S2 = 40
+    a2 = rng.random((T, S2)).astype(np.float32)
+    r2 = rng.random((T, S2)).astype(np.float32)
+    sts = lambda x: np.ascontiguousarray(x.reshape(T, S2 // 8, 8).transpose(1, 0, 2))
+    o = [np.full(T - 5, -7, np.float32)]
+    a2s, r2s = sts(a2), sts(r2)   # kept alive across the call
+    H.ref_corr_with(1, 0, 1, ptrs([a2s]), r2s.ctypes.data, ptrs(o), S2, T, 5, T - 5)
+    out.update(cw_sts_a=a2, cw_sts_r=r2, cw_sts=o[0])
+    # aggregrate: 19 stocks, 200 rows in 9 label groups of uneven length,
+    # NaN cells inside groups
+    for dt, name in ((np.float32, "f32"), (np.float64, "f64")):
+        S3, T3 = 19, 200
+        x = (rng.standard_normal((T3, S3)) * 10).astype(dt)
+        x[17, 3] = np.nan
+        x[101:104, 5] = np.nan
+        cuts = np.sort(rng.choice(np.arange(1, T3), 8, replace=False))
+        labels = np.zeros(T3, dt)
+        for c in cuts:
+            labels[c:] += 1
+        groups = 9
+        outs = [np.full((groups, S3), -7, dt) for _ in AGG_KINDS]
+        H.ref_aggregrate(1, ptrs([x]), ptrs([labels]), 1 if dt == np.float64 else 0, ptrs(outs), S3, T3, 0, T3)
+        out[f"agg_{name}_x"] = x
+        out[f"agg_{name}_labels"] = labels
+        for k, v in zip(AGG_KINDS, outs):
+            out[f"agg_{name}_{k}"] = v
+    np.savez_compressed(os.path.join(HERE, "ref_rowreduce.npz"), **out)
+    print("wrote rowreduce", len(out), "arrays")
+
+
 def main():
     if not os.path.exists(os.path.join(REF_OUT, "libKunRuntime.so")):
         subprocess.check_call(["bash", os.path.join(ROOT, "oracle", "build_ref.sh")])
+    if "--rowreduce" in sys.argv:
+        rowreduce_vectors()
+        return
     K = ref_namespace()
     if "--bench-modules" in sys.argv:
         # reference builds of bench graphs with no fixture: bench.py times

@@ -95,3 +95,35 @@ def test_oracle_rank_of_rank_matches_reference():
     for k in ("r2", "wsum", "t60", "t30", "f022"):
         assert same_values(got[k][:, st], z[f"out_{k}"]), k
     assert np.nanmax(z["out_t30"]) > 30.5
+
+
+def _rowreduce():
+    return np.load(os.path.join(GOLDEN, "ref_rowreduce.npz"))
+
+
+@pytest.mark.parametrize("rank_inputs", [False, True])
+def test_oracle_corrwith_matches_reference(rank_inputs):
+    """corrWith over TS rows (37 stocks, NaN cells, a NaN row, ties) and
+    STs rows against the reference runtime's own corrWith
+    (tests/golden/ref_rowreduce.npz), bit for bit."""
+    from oracle import kun_oracle as ko
+    z = _rowreduce()
+    exp = z[f"cw_ts_rank{int(rank_inputs)}"]
+    for i, k in enumerate(("cw_a", "cw_b")):
+        got = ko.corr_with(z[k], z["cw_r"], rank_inputs=rank_inputs)
+        assert same_values(got, exp[i]), k
+    if not rank_inputs:
+        got = ko.corr_with(z["cw_sts_a"][5:], z["cw_sts_r"][5:])
+        assert same_values(got, z["cw_sts"])
+
+
+@pytest.mark.parametrize("dt", ["f32", "f64"])
+def test_oracle_aggregrate_matches_reference(dt):
+    """aggregrate (sum, min, max, first, last, count, mean over runs of
+    equal labels, NaN cells inside groups) against the reference runtime's
+    own aggregrate, bit for bit."""
+    from oracle import kun_oracle as ko
+    z = _rowreduce()
+    got = ko.aggregate(z[f"agg_{dt}_x"], z[f"agg_{dt}_labels"])
+    for k in ("sum", "min", "max", "first", "last", "count", "mean"):
+        assert same_values(got[k], z[f"agg_{dt}_{k}"]), k

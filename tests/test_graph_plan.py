@@ -8,8 +8,10 @@ only on nodes added before it (so the graph is acyclic and every handle
 exists when used); the nodes of one job depend only on that job's nodes;
 each launch-DAG node maps to a chain of kernels (two for a Scale stage: row
 sums, then the division); the first kernel of a DAG node depends on exactly
-the last kernel of each of its DAG dependencies, later kernels on the one
-before them; every DAG node is present once."""
+the last kernel of each of its DAG dependencies and of the DAG node before it
+on the same lane (the arena's memory plan reuses slots along lane order, as
+the executor's streams serialise it), later kernels on the one before them;
+every DAG node is present once."""
 import ctypes
 from collections import defaultdict
 
@@ -26,8 +28,8 @@ def parse(text):
         deps = [int(x) for x in deps.split()]
         f = head.split()
         if f[0] == "dag":
-            job, n, stage, kern = map(int, f[1:5])
-            dag[job][n] = (stage, kern, deps)
+            job, n, stage, kern, lane = map(int, f[1:6])
+            dag[job][n] = (stage, kern, lane, deps)
         else:
             nid, kind, job, stage, kern, grid, block = int(f[0]), f[1], *map(int, f[2:7])
             nodes[nid] = dict(kind=kind, job=job, stage=stage, kern=kern, grid=grid, block=block, deps=deps)
@@ -59,11 +61,15 @@ def check_graph(text, mods):
         kinds = {i: st[1] for i, st in enumerate(mod.stages())}
         got = chains[job]
         assert len(got) == len(dnodes), f"job {job}: {len(got)} node chains for {len(dnodes)} DAG nodes"
-        last_of = {}
+        last_of, prev_on_lane = {}, {}
         for n, (key, chain) in enumerate(got):
-            stage, kern, deps = dnodes[n]
+            stage, kern, lane, deps = dnodes[n]
             assert key == (job, stage, kern), f"job {job} DAG node {n}: graph chain {key}"
-            want = sorted(last_of[d] for d in deps)
+            preds = set(deps)
+            if lane in prev_on_lane:
+                preds.add(prev_on_lane[lane])
+            prev_on_lane[lane] = n
+            want = sorted(last_of[d] for d in preds)
             assert sorted(nodes[chain[0]]["deps"]) == want, f"job {job} DAG node {n}: deps"
             for a, b in zip(chain, chain[1:]):
                 assert nodes[b]["deps"] == [a]

@@ -47,3 +47,27 @@ def test_factor_library_matches_reference(golden, module):
     assert not bad, f"{len(bad)} of {len(outs)} factors differ: {bad[:8]}"
     # the stored rows hold values, not only warm-up NaNs
     assert sum(np.isfinite(v).any() for v in outs.values()) >= 0.9 * len(outs)
+
+
+def test_alpha101_through_the_kunquant_alias():
+    """Code written against the reference's package names (the KunQuant
+    alias of kunquant_amd/compat): Library.load / getModule / runGraph of
+    the Alpha101 module give the reference's outputs."""
+    import importlib
+    import sys
+    compat = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "kunquant_amd", "compat")
+    sys.path.insert(0, compat)
+    try:
+        kr = importlib.import_module("KunQuant.runner.KunRunner")
+        z = np.load(os.path.join(GOLD, "ref_alpha101_f32.npz"))
+        ins = {k[3:]: z[k] for k in z.files if k.startswith("in_")}
+        meta = [int(x) for x in z["meta"]]
+        lib = kr.Library.load(catalog.library_path("alpha101_f32"))
+        mod = lib.getModule("alpha101_f32")
+        executor = kr.createMultiThreadExecutor(8)
+        out = kr.runGraph(executor, mod, ins, meta[2], meta[3])
+        bad = [k for k in z.files if k.startswith("out_")
+               and not np.array_equal(out[k[4:]][meta[4]:], z[k], equal_nan=True)]
+        assert not bad, bad[:5]
+    finally:
+        sys.path.remove(compat)

@@ -344,8 +344,8 @@ def rank_kernel_name(np_dt, S):
     for lim, block, items in ((256, 256, 1), (512, 256, 2), (1024, 256, 4), (2048, 512, 4), (4096, 1024, 4),
                               (8192, 1024, 8), (16384, 1024, 16)):
         if S <= lim:
-            keys_only = "true" if (items == 16 and t == "double") else "false"
-            return f"rank_rows<{t}, {block}, {items}, {keys_only}>"
+            kind = 1 if (items == 16 and t == "double") else 0  # 1: keys-only radix sort
+            return f"rank_rows<{t}, {block}, {items}, {kind}>"
     return "rank_rows"
 
 

@@ -6,8 +6,8 @@
 // T(sum) / T(hi - lo) / T(n). -0 and +0 tie (std::sort / equal_range use
 // operator<). On the device a row's values become order-preserving integer
 // keys (-0 canonicalised to +0, NaN to the largest key) and are sorted with
-// (stock) payloads by an LSD radix sort in LDS written for 64-lane
-// wavefronts (below); tie groups are read off the sorted order.
+// their stocks by an LSD radix sort in LDS, 4-bit digits counted per lane
+// (CountRank, below); tie groups are read off the sorted order.
 //
 // Scale (restates cpp/Kun/Scale.hpp:13-46): sum of |x| over non-NaN values
 // accumulated sequentially in stock order (bit-exact order), then
@@ -119,13 +119,25 @@ template <typename V> __device__ __forceinline__ V wave_or(V v) {
     return v;
 }
 
-// ---- Rank: LSD radix sort of one row in LDS -----------------------------
+// rank of every member of the tie group at sorted positions [a, b) among n
+// non-NaN values
+template <typename T> __device__ __forceinline__ T group_rank(int a, int b, int n) {
+    // rows of at most 16384 stocks: (a + b + 1) * (b - a) < 2^31, so the
+    // reference's integer sum fits 32 bits and converts to T the same way
+    const int32_t sum = (a + b + 1) * (b - a) / 2;
+    // x / 1 == x exactly: a group of one skips the first division
+    const T q = (b - a == 1) ? (T)sum : (T)sum / (T)(b - a);
+    return q / (T)n;
+}
+
+// ---- Rank of long 64-bit rows: keys-only LSD radix sort -------------------
 //
-// A workgroup of BLOCK lanes (NW = BLOCK / 64 wavefronts) holds N = BLOCK x
-// ITEMS keys in registers, wavefront w owning positions [w*64*ITEMS,
-// (w+1)*64*ITEMS): item j of lane l is position w*64*ITEMS + j*64 + l
-// (loads and stores of consecutive lanes are consecutive). Each pass sorts
-// stably by one 8-bit digit:
+// For 64-bit keys of rows above 8192 stocks, whose (key, stock) pairs would
+// not fit in LDS. A workgroup of BLOCK lanes (NW = BLOCK / 64 wavefronts)
+// holds N = BLOCK x ITEMS keys in registers, wavefront w owning positions
+// [w*64*ITEMS, (w+1)*64*ITEMS): item j of lane l is position
+// w*64*ITEMS + j*64 + l (loads and stores of consecutive lanes are
+// consecutive). Each pass sorts stably by one 8-bit digit:
 //  1. ranking: per item, the lanes with the same digit are found by ballots
 //     (same_digit); the lowest such lane adds their count to the wavefront's
 //     own LDS histogram row, every lane's rank within its digit being the
@@ -133,27 +145,22 @@ template <typename V> __device__ __forceinline__ V wave_or(V v) {
 //  2. a block scan turns the histograms into starting positions, digit by
 //     digit and within a digit wavefront by wavefront -- the order the
 //     positions held -- so the pass is stable;
-//  3. keys (and their stocks) are scattered to LDS and read back.
+//  3. keys are scattered to LDS and read back.
 // Digit positions on which every key of the row agrees are skipped (their
 // stable pass would keep the order); the row's AND and OR of keys tell.
-// Pairs form: keys carry their stock (16 bits) and tie groups come from a
-// max-scan of group starts in the sorted order. Keys-only form (64-bit keys
-// of rows above 8192 stocks, where pairs would not fit in LDS): each lane
-// finds its own keys' tie groups by binary search in the sorted keys.
-template <typename T, int BLOCK, int ITEMS, bool KEYS_ONLY> struct RadixRank {
+// Each lane then finds its own keys' tie groups by binary search in the
+// sorted keys.
+template <typename T, int BLOCK, int ITEMS> struct KeysRank {
     using K = typename KeyOf<T>::K;
     static constexpr int N = BLOCK * ITEMS;
     static constexpr int NW = BLOCK / 64;
     static_assert(NW % 4 == 0 && NW <= 16, "BLOCK must be 256, 512 or 1024");
-    static_assert(KEYS_ONLY || N <= 65536, "stock payloads are 16-bit");
     struct Smem {
-        K key[N];                        // keys; then group ends (int); then ranks by stock (T)
-        uint16_t idx[KEYS_ONLY ? 1 : N]; // stock of the key at each position
+        K key[N];                        // keys in sorted order
         uint32_t hist[NW][256];          // per wavefront digit counts, then starting positions
         uint32_t wv[NW];                 // per wavefront totals of the scans
         K wand[NW], wor[NW];
     };
-    static_assert(sizeof(K) >= sizeof(int) && sizeof(K) >= sizeof(T), "LDS aliasing");
 
     // hist := exclusive prefix in (digit, wavefront) order. Thread t owns
     // digit t / (NW/4) of wavefronts 4g .. 4g+3 (g = t % (NW/4)): thread
@@ -188,8 +195,6 @@ template <typename T, int BLOCK, int ITEMS, bool KEYS_ONLY> struct RadixRank {
         T *dst = (T *)out.ptr;
         const long rin = t + in.row0;
         K key[ITEMS];
-        // stock of each key, two 16-bit ids per register
-        uint32_t id2[KEYS_ONLY ? 1 : (ITEMS + 1) / 2];
         int cnt = 0;
         K kand = ~K(0), kor = K(0);
         // every load issued before any is used (stocks past S read the last
@@ -207,10 +212,6 @@ template <typename T, int BLOCK, int ITEMS, bool KEYS_ONLY> struct RadixRank {
             const K k = live ? rank_key(xs[j]) : ~K(0);
             cnt += (live && xs[j] == xs[j]) ? 1 : 0;
             key[j] = k;
-            if constexpr (!KEYS_ONLY) {
-                if (j % 2 == 0) id2[j / 2] = (uint32_t)s;
-                else id2[j / 2] |= (uint32_t)s << 16;
-            }
             kand &= k;
             kor |= k;
         }
@@ -266,18 +267,10 @@ template <typename T, int BLOCK, int ITEMS, bool KEYS_ONLY> struct RadixRank {
             for (int j = 0; j < ITEMS; j++) {
                 const uint32_t pos = hw[(uint32_t)(key[j] >> sh) & 0xffu] + ((loc2[j / 2] >> (16 * (j % 2))) & 0xffffu);
                 sm.key[pos] = key[j];
-                if constexpr (!KEYS_ONLY) sm.idx[pos] = (uint16_t)(id2[j / 2] >> (16 * (j % 2)));
             }
             __syncthreads();
 #pragma unroll
-            for (int j = 0; j < ITEMS; j++) {
-                key[j] = sm.key[base + j * 64];
-                if constexpr (!KEYS_ONLY) {
-                    const uint32_t v = sm.idx[base + j * 64];
-                    if (j % 2 == 0) id2[j / 2] = v;
-                    else id2[j / 2] |= v << 16;
-                }
-            }
+            for (int j = 0; j < ITEMS; j++) key[j] = sm.key[base + j * 64];
         }
         if (!moved) {
             // every key equal: the order is sorted as it stands
@@ -285,79 +278,19 @@ template <typename T, int BLOCK, int ITEMS, bool KEYS_ONLY> struct RadixRank {
             for (int j = 0; j < ITEMS; j++) sm.key[base + j * 64] = key[j];
         }
         __syncthreads(); // the sorted keys are in sm.key
-        if constexpr (KEYS_ONLY) {
-            // this lane's own keys again (the registers now hold sorted ones)
+        // this lane's own keys again (the registers now hold sorted ones)
 #pragma unroll
-            for (int j = 0; j < ITEMS; j++) {
-                const long s = base + j * 64;
-                key[j] = rank_key(src[cs_at(in.layout, rin, s < S ? s : S - 1, S, in.rows, blk)]);
-            }
-#pragma unroll
-            for (int j = 0; j < ITEMS; j++)
-                if (base + j * 64 >= S) key[j] = ~K(0);
-            rank_by_search(sm, key, n, dst, out, S, blk, t, base);
-        } else {
-            rank_by_groups(sm, key, id2, n, dst, out, S, blk, t, base);
+        for (int j = 0; j < ITEMS; j++) {
+            const long s = base + j * 64;
+            key[j] = rank_key(src[cs_at(in.layout, rin, s < S ? s : S - 1, S, in.rows, blk)]);
         }
+#pragma unroll
+        for (int j = 0; j < ITEMS; j++)
+            if (base + j * 64 >= S) key[j] = ~K(0);
+        rank_by_search(sm, key, n, dst, out, S, blk, t, base);
     }
 
-    static __device__ __forceinline__ T rank_of(int a, int b, int n) {
-        // rows of at most 16384 stocks: (a + b + 1) * (b - a) < 2^31, so the
-        // reference's integer sum fits 32 bits and converts to T the same way
-        const int32_t sum = (a + b + 1) * (b - a) / 2;
-        // x / 1 == x exactly: a group of one skips the first division
-        const T q = (b - a == 1) ? (T)sum : (T)sum / (T)(b - a);
-        return q / (T)n;
-    }
-
-    static __device__ __forceinline__ void rank_by_groups(Smem &sm, const K (&key)[ITEMS],
-                                                          const uint32_t (&id2)[(ITEMS + 1) / 2],
-                                                          int n, T *dst, const kun_cs::View &out, long S, int blk,
-                                                          long t, int base) {
-        const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-        // group starts: a(P) = the last P' <= P starting a group (a max-scan
-        // over positions in (wavefront, item, lane) order)
-        int a[ITEMS];
-        bool last[ITEMS];
-        int carry = 0;
-#pragma unroll
-        for (int j = 0; j < ITEMS; j++) {
-            const int P = base + j * 64;
-            const bool first = P == 0 || sm.key[P - 1] != key[j];
-            last[j] = P + 1 == N || sm.key[P + 1] != key[j];
-            int v = wave_incl_max(first ? P : 0);
-            v = v > carry ? v : carry;
-            a[j] = v;
-            carry = __shfl(v, 63, 64);
-        }
-        if (lane == 63) sm.wv[w] = (uint32_t)carry;
-        __syncthreads(); // also: every neighbour read of sm.key is done
-        int pre = 0;
-        for (int q = 0; q < w; q++) pre = (int)sm.wv[q] > pre ? (int)sm.wv[q] : pre;
-        int *endof = (int *)sm.key;
-#pragma unroll
-        for (int j = 0; j < ITEMS; j++) {
-            a[j] = a[j] > pre ? a[j] : pre;
-            const int P = base + j * 64;
-            if (P < n && last[j]) endof[a[j]] = P + 1;
-        }
-        __syncthreads();
-        T r[ITEMS];
-#pragma unroll
-        for (int j = 0; j < ITEMS; j++) {
-            const int P = base + j * 64;
-            r[j] = P < n ? rank_of(a[j], endof[a[j]], n) : kun::qnan<T>();
-        }
-        __syncthreads();
-        T *res = (T *)sm.key;
-#pragma unroll
-        for (int j = 0; j < ITEMS; j++) {
-            const int id = (int)((id2[j / 2] >> (16 * (j % 2))) & 0xffffu);
-            if (id < S) res[id] = r[j];
-        }
-        __syncthreads();
-        for (long s = threadIdx.x; s < S; s += BLOCK) dst[cs_at(out.layout, t + out.row0, s, S, out.rows, blk)] = res[s];
-    }
+    static __device__ __forceinline__ T rank_of(int a, int b, int n) { return group_rank<T>(a, b, n); }
 
     static __device__ __forceinline__ void rank_by_search(Smem &sm, const K (&mine)[ITEMS], int n, T *dst,
                                                           const kun_cs::View &out, long S, int blk, long t,
@@ -425,26 +358,301 @@ template <typename T, int BLOCK, int ITEMS, bool KEYS_ONLY> struct RadixRank {
     }
 };
 
-template <typename T, int BLOCK, int ITEMS, bool KEYS_ONLY>
-__device__ __forceinline__ void rank_row(const kun_cs::View &in, const kun_cs::View &out, long S, int blk,
-                                         const long t) {
-    using R = RadixRank<T, BLOCK, ITEMS, KEYS_ONLY>;
-    __shared__ typename R::Smem sm;
-    R::row(sm, in, out, S, blk, t);
+// inclusive prefix sum / max over the 64 lanes by DPP row shifts and row
+// broadcasts (rows of 16 lanes, then across rows); max assumes values >= 0
+__device__ __forceinline__ uint32_t dpp_incl_sum(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false); // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false); // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false); // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false); // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false); // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false); // row_bcast:31
+    return v;
+}
+// inclusive prefix sum within each row of 16 lanes
+__device__ __forceinline__ uint32_t dpp_row_incl_sum(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false); // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false); // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false); // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false); // row_shr:8
+    return v;
+}
+__device__ __forceinline__ int dpp_incl_max(int v) {
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false));
+    return v;
+}
+
+// ---- Rank: LSD radix sort with per-lane digit counters -------------------
+//
+// The (key, stock) form for rows whose pairs fit in LDS. A workgroup of
+// BLOCK lanes holds N = BLOCK x ITEMS keys in registers, lane t owning
+// positions [t*ITEMS, (t+1)*ITEMS) (blocked; the first pass may start from
+// any order, the loads are striped for coalescing). Each pass sorts stably by
+// one 4-bit digit:
+//  1. each lane counts its own keys' digits in registers, sixteen 4-bit
+//     fields per 8 keys: a key's rank among the lane's earlier keys of its
+//     digit is the fields' value before its increment. The counts go to the
+//     lane's own column of the LDS table cnt[8][BLOCK], digit d in half
+//     d >> 3 of word cnt[d & 7][t] -- no atomics, no ballots;
+//  2. one block scan of the table in memory order (word-major, lane-minor:
+//     digits 0-7 in the low halves, 8-15 in the high halves; adding the low
+//     digits' total to every high half puts digits 8-15 after them) gives
+//     each (digit, lane) the position of its first key;
+//  3. keys and stocks are scattered to their positions in LDS and read back
+//     blocked.
+// Within a digit, positions go lane by lane and within a lane key by key --
+// the order the positions held -- so each pass is stable. Digit positions
+// on which every key of the row agrees are skipped (the row's AND and OR of
+// keys tell). Tie groups come from a max-scan of group starts over the
+// sorted positions.
+template <typename T, int BLOCK, int ITEMS> struct CountRank {
+    using K = typename KeyOf<T>::K;
+    static constexpr int N = BLOCK * ITEMS;
+    static constexpr int NW = BLOCK / 64;
+    static constexpr int NG = (ITEMS + 7) / 8; // groups of up to 8 keys, one field register each
+    static constexpr int NB = (int)sizeof(K) * 8;
+    static_assert(BLOCK % 64 == 0 && NW <= 16, "BLOCK: 64..1024 lanes");
+    static_assert(N <= 16384, "positions and counts are 16-bit");
+    struct alignas(16) Smem {
+        K key[N];                // sorted keys; scratch of the block scans; then group ends, ranks by stock
+        uint16_t idx[N];         // stock at each position
+        uint32_t cnt[8 * BLOCK]; // digit counts, then first positions; scratch of the group scan
+    };
+
+    static __device__ __forceinline__ uint32_t digit(K k, int sh) { return (uint32_t)(k >> sh) & 15u; }
+
+    static __device__ __forceinline__ void row(Smem &sm, const kun_cs::View &in, const kun_cs::View &out, long S,
+                                               int blk, long t) {
+        if (S <= 0) return; // uniform: nothing to rank
+        const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+        const int wu = __builtin_amdgcn_readfirstlane(w); // w, known uniform
+        const T *src = (const T *)in.ptr;
+        T *dst = (T *)out.ptr;
+        const long rin = t + in.row0;
+        T xs[ITEMS];
+#pragma unroll
+        for (int j = 0; j < ITEMS; j++) {
+            const long s = w * 64 * ITEMS + j * 64 + lane;
+            xs[j] = src[cs_at(in.layout, rin, s < S ? s : S - 1, S, in.rows, blk)];
+        }
+        K key[ITEMS];
+        uint32_t id2[(ITEMS + 1) / 2]; // stock of each key, two 16-bit ids per register
+        int cnt = 0;
+        K kand = ~K(0), kor = K(0);
+#pragma unroll
+        for (int j = 0; j < ITEMS; j++) {
+            const long s = w * 64 * ITEMS + j * 64 + lane;
+            const bool live = s < S;
+            const K k = live ? rank_key(xs[j]) : ~K(0);
+            cnt += (live && xs[j] == xs[j]) ? 1 : 0;
+            key[j] = k;
+            if (j % 2 == 0) id2[j / 2] = (uint32_t)s;
+            else id2[j / 2] |= (uint32_t)s << 16;
+            kand &= k;
+            kor |= k;
+        }
+        cnt = wave_sum(cnt);
+        kand = wave_and(kand);
+        kor = wave_or(kor);
+        // wavefront partials in the key area (its next writer, the first
+        // pass's scan, comes after a barrier)
+        K *xa = sm.key, *xo = sm.key + NW;
+        uint32_t *xc = (uint32_t *)(sm.key + 2 * NW);
+        if (lane == 0) {
+            xa[w] = kand;
+            xo[w] = kor;
+            xc[w] = (uint32_t)cnt;
+        }
+        __syncthreads();
+        // lane q < NW holds wavefront q's partials
+        const int n = wave_sum(lane < NW ? (int)xc[lane] : 0);
+        kand = wave_and(lane < NW ? xa[lane] : ~K(0));
+        kor = wave_or(lane < NW ? xo[lane] : K(0));
+        const K diff = kand ^ kor; // bits on which the row's keys differ
+        const uint16_t *first16 = (const uint16_t *)sm.cnt;
+        uint32_t *xs32 = (uint32_t *)sm.key; // scan scratch
+        bool moved = false;
+        for (int sh = 0; sh < NB; sh += 4) {
+            if (digit(diff, sh) == 0u) continue; // uniform over the workgroup
+            moved = true;
+            // 1. per-lane counts; nibble j % 8 of pre[j / 8]: the earlier keys
+            // of this lane with key j's digit (at most 15)
+            uint64_t f[NG];
+            uint32_t pre[NG];
+#pragma unroll
+            for (int g = 0; g < NG; g++) {
+                f[g] = 0;
+                pre[g] = 0;
+            }
+#pragma unroll
+            for (int j = 0; j < ITEMS; j++) {
+                const uint32_t d4 = digit(key[j], sh) * 4u;
+                uint32_t p = 0;
+#pragma unroll
+                for (int g = 0; g < j / 8; g++) p += (uint32_t)(f[g] >> d4) & 15u;
+                p += (uint32_t)(f[j / 8] >> d4) & 15u;
+                pre[j / 8] |= p << (4 * (j % 8));
+                f[j / 8] += (uint64_t)1 << d4;
+            }
+            // the packed prefixes stay packed (eight 4-bit fields in one
+            // register) rather than being recomputed at the scatter from
+            // eight 64-bit partial counts
+#pragma unroll
+            for (int g = 0; g < NG; g++) asm volatile("" : "+v"(pre[g]));
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                uint32_t word = 0;
+#pragma unroll
+                for (int g = 0; g < NG; g++)
+                    word += (((uint32_t)f[g] >> (4 * k)) & 15u) | ((((uint32_t)(f[g] >> 32) >> (4 * k)) & 15u) << 16);
+                sm.cnt[k * BLOCK + tid] = word;
+            }
+            __syncthreads();
+            // 2. exclusive scan of the table in memory order; lane t owns words [8t, 8t+8)
+            uint4 *c4 = (uint4 *)sm.cnt;
+            const uint4 lo = c4[2 * tid], hi = c4[2 * tid + 1];
+            const uint32_t v[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+            uint32_t s = 0;
+#pragma unroll
+            for (int i = 0; i < 8; i++) s += v[i];
+            const uint32_t incl = dpp_incl_sum(s);
+            if (lane == 63) xs32[w] = incl;
+            __syncthreads();
+            // wavefront totals: lane q of every wavefront holds the sum of
+            // totals 0..q (prefix over the first 16 lanes)
+            const uint32_t wp = dpp_row_incl_sum(lane < NW ? xs32[lane] : 0u);
+            const uint32_t total = (uint32_t)__builtin_amdgcn_readlane((int)wp, NW - 1);
+            const uint32_t before = wu ? (uint32_t)__builtin_amdgcn_readlane((int)wp, wu - 1) : 0u;
+            uint32_t run = before + incl - s + ((total & 0xffffu) << 16);
+            uint32_t o[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) {
+                o[i] = run;
+                run += v[i];
+            }
+            c4[2 * tid] = make_uint4(o[0], o[1], o[2], o[3]);
+            c4[2 * tid + 1] = make_uint4(o[4], o[5], o[6], o[7]);
+            __syncthreads();
+            // 3. scatter to the sorted positions (every lookup issued before
+            // the first store), read back blocked
+            uint32_t pos[ITEMS];
+#pragma unroll
+            for (int j = 0; j < ITEMS; j++) {
+                const uint32_t d = digit(key[j], sh);
+                pos[j] = first16[((d & 7u) * BLOCK + tid) * 2 + (d >> 3)];
+            }
+#pragma unroll
+            for (int j = 0; j < ITEMS; j++) {
+                const uint32_t at = pos[j] + ((pre[j / 8] >> (4 * (j % 8))) & 15u);
+                sm.key[at] = key[j];
+                sm.idx[at] = (uint16_t)(id2[j / 2] >> (16 * (j % 2)));
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < ITEMS; j++) key[j] = sm.key[tid * ITEMS + j];
+            if constexpr (ITEMS % 2 == 0) {
+                // stock pairs (2j, 2j+1) are adjacent: one word each
+#pragma unroll
+                for (int j = 0; j < ITEMS / 2; j++) id2[j] = ((const uint32_t *)sm.idx)[tid * ITEMS / 2 + j];
+            } else {
+                id2[0] = sm.idx[tid];
+            }
+        }
+        if (!moved) {
+            // every key equal: any order is sorted
+            __syncthreads(); // the partials' readers are done
+#pragma unroll
+            for (int j = 0; j < ITEMS; j++) sm.key[tid * ITEMS + j] = key[j];
+            __syncthreads();
+        }
+        ties(sm, key, id2, n, dst, out, S, blk, t);
+    }
+
+    static __device__ __forceinline__ T rank_of(int a, int b, int n) { return group_rank<T>(a, b, n); }
+
+    // sorted keys at positions P = tid*ITEMS + j (registers and sm.key)
+    static __device__ __forceinline__ void ties(Smem &sm, const K (&key)[ITEMS], const uint32_t (&id2)[(ITEMS + 1) / 2],
+                                                int n, T *dst, const kun_cs::View &out, long S, int blk, long t) {
+        const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+        const int P0 = tid * ITEMS;
+        // a[j]: start of key j's group (the last position <= P0 + j that starts one)
+        int a[ITEMS];
+        bool last[ITEMS];
+        int cur = -1;
+#pragma unroll
+        for (int j = 0; j < ITEMS; j++) {
+            const bool first = j > 0 ? key[j - 1] != key[j] : (P0 == 0 || sm.key[P0 - 1] != key[0]);
+            last[j] = j + 1 < ITEMS ? key[j] != key[j + 1] : (P0 + ITEMS == N || sm.key[P0 + ITEMS] != key[j]);
+            if (first) cur = P0 + j;
+            a[j] = cur;
+        }
+        // group start carried in from earlier lanes: exclusive max-scan of
+        // (cur + 1), 0 meaning none
+        const int incl = dpp_incl_max(cur + 1);
+        int prev = __shfl_up(incl, 1, 64);
+        if (lane == 0) prev = 0;
+        int *xm = (int *)sm.cnt;
+        if (lane == 63) xm[w] = incl;
+        __syncthreads(); // also: every read of sm.key above is done
+        int carry = prev;
+#pragma unroll
+        for (int q = 0; q < NW; q++) carry = q < w && xm[q] > carry ? xm[q] : carry;
+        carry -= 1;
+        int *endof = (int *)sm.key;
+#pragma unroll
+        for (int j = 0; j < ITEMS; j++) {
+            if (a[j] < 0) a[j] = carry;
+            const int P = P0 + j;
+            if (P < n && last[j]) endof[a[j]] = P + 1;
+        }
+        __syncthreads();
+        T r[ITEMS];
+#pragma unroll
+        for (int j = 0; j < ITEMS; j++) r[j] = P0 + j < n ? rank_of(a[j], endof[a[j]], n) : kun::qnan<T>();
+        __syncthreads();
+        T *res = (T *)sm.key;
+#pragma unroll
+        for (int j = 0; j < ITEMS; j++) {
+            const int id = (int)((id2[j / 2] >> (16 * (j % 2))) & 0xffffu);
+            if (id < S) res[id] = r[j];
+        }
+        __syncthreads();
+        for (long s = tid; s < S; s += BLOCK) dst[cs_at(out.layout, t + out.row0, s, S, out.rows, blk)] = res[s];
+    }
+};
+
+// KIND 0: CountRank (pairs); 1: KeysRank (64-bit keys beyond 8192 stocks)
+template <typename T, int BLOCK, int ITEMS, int KIND>
+__device__ __forceinline__ void rank_row_k(const kun_cs::View &in, const kun_cs::View &out, long S, int blk,
+                                           const long t) {
+    if constexpr (KIND == 0) {
+        using R = CountRank<T, BLOCK, ITEMS>;
+        __shared__ typename R::Smem sm;
+        R::row(sm, in, out, S, blk, t);
+    } else {
+        using R = KeysRank<T, BLOCK, ITEMS>;
+        __shared__ typename R::Smem sm;
+        R::row(sm, in, out, S, blk, t);
+    }
 }
 
 // one workgroup per row of the views
-template <typename T, int BLOCK, int ITEMS, bool KEYS_ONLY>
+template <typename T, int BLOCK, int ITEMS, int KIND>
 __global__ __launch_bounds__(BLOCK) void rank_rows(kun_cs::View in, kun_cs::View out, long S, int blk) {
-    rank_row<T, BLOCK, ITEMS, KEYS_ONLY>(in, out, S, blk, blockIdx.x);
+    rank_row_k<T, BLOCK, ITEMS, KIND>(in, out, S, blk, blockIdx.x);
 }
 
 // one workgroup per segment (a contiguous run of stocks of one row)
-template <typename T, int BLOCK, int ITEMS, bool KEYS_ONLY>
+template <typename T, int BLOCK, int ITEMS, int KIND>
 __global__ __launch_bounds__(BLOCK) void rank_segs(const kun_cs::Seg *segs) {
     const kun_cs::Seg g = segs[blockIdx.x];
     const kun_cs::View in{const_cast<void *>(g.in), 2, 1, 0}, out{g.out, 2, 1, 0};
-    rank_row<T, BLOCK, ITEMS, KEYS_ONLY>(in, out, g.n, 1, 0);
+    rank_row_k<T, BLOCK, ITEMS, KIND>(in, out, g.n, 1, 0);
 }
 
 // ROWS rows per workgroup; the sum of each row is one sequential chain (lane
@@ -589,41 +797,42 @@ hipError_t launch_scale_t(const kun_cs::View &in, const kun_cs::View &out, long 
     return sink_launch(sink, scale_rows<T, -1, -1>, grid, dim3(SCALE_BLOCK), in, out, S, rows, blk);
 }
 
-template <typename T, int BLOCK, int ITEMS, bool KEYS_ONLY>
+template <typename T, int BLOCK, int ITEMS, int KIND>
 hipError_t launch_rank_n(const kun_cs::View &in, const kun_cs::View &out, long S, long rows, int blk,
                          kun_cs::Sink &sink) {
-    return sink_launch(sink, rank_rows<T, BLOCK, ITEMS, KEYS_ONLY>, dim3((unsigned)rows), dim3(BLOCK), in, out, S, blk);
+    return sink_launch(sink, rank_rows<T, BLOCK, ITEMS, KIND>, dim3((unsigned)rows), dim3(BLOCK), in, out, S, blk);
 }
 
 // Workgroup shape by row length: at most 1024 lanes, up to 16 keys per lane;
-// 64-bit keys of more than 8192 stocks sort without payloads (LDS).
+// 64-bit keys of more than 8192 stocks sort without payloads (their pairs
+// would not fit in LDS).
 template <typename T>
 hipError_t launch_rank_t(const kun_cs::View &in, const kun_cs::View &out, long S, long rows, int blk,
                          kun_cs::Sink &sink) {
-    if (S <= 256) return launch_rank_n<T, 256, 1, false>(in, out, S, rows, blk, sink);
-    if (S <= 512) return launch_rank_n<T, 256, 2, false>(in, out, S, rows, blk, sink);
-    if (S <= 1024) return launch_rank_n<T, 256, 4, false>(in, out, S, rows, blk, sink);
-    if (S <= 2048) return launch_rank_n<T, 512, 4, false>(in, out, S, rows, blk, sink);
-    if (S <= 4096) return launch_rank_n<T, 1024, 4, false>(in, out, S, rows, blk, sink);
-    if (S <= 8192) return launch_rank_n<T, 1024, 8, false>(in, out, S, rows, blk, sink);
-    if (S <= 16384) return launch_rank_n<T, 1024, 16, sizeof(T) == 8>(in, out, S, rows, blk, sink);
+    if (S <= 256) return launch_rank_n<T, 256, 1, 0>(in, out, S, rows, blk, sink);
+    if (S <= 512) return launch_rank_n<T, 256, 2, 0>(in, out, S, rows, blk, sink);
+    if (S <= 1024) return launch_rank_n<T, 256, 4, 0>(in, out, S, rows, blk, sink);
+    if (S <= 2048) return launch_rank_n<T, 512, 4, 0>(in, out, S, rows, blk, sink);
+    if (S <= 4096) return launch_rank_n<T, 1024, 4, 0>(in, out, S, rows, blk, sink);
+    if (S <= 8192) return launch_rank_n<T, 1024, 8, 0>(in, out, S, rows, blk, sink);
+    if (S <= 16384) return launch_rank_n<T, 1024, 16, sizeof(T) == 8 ? 1 : 0>(in, out, S, rows, blk, sink);
     return hipErrorInvalidValue;
 }
 
-template <typename T, int BLOCK, int ITEMS, bool KEYS_ONLY>
+template <typename T, int BLOCK, int ITEMS, int KIND>
 hipError_t launch_rank_segs_n(const kun_cs::Seg *segs, int nseg, hipStream_t st) {
-    hipLaunchKernelGGL((rank_segs<T, BLOCK, ITEMS, KEYS_ONLY>), dim3((unsigned)nseg), dim3(BLOCK), 0, st, segs);
+    hipLaunchKernelGGL((rank_segs<T, BLOCK, ITEMS, KIND>), dim3((unsigned)nseg), dim3(BLOCK), 0, st, segs);
     return hipGetLastError();
 }
 
 template <typename T> hipError_t launch_rank_segs_t(const kun_cs::Seg *segs, int nseg, long maxS, hipStream_t st) {
-    if (maxS <= 256) return launch_rank_segs_n<T, 256, 1, false>(segs, nseg, st);
-    if (maxS <= 512) return launch_rank_segs_n<T, 256, 2, false>(segs, nseg, st);
-    if (maxS <= 1024) return launch_rank_segs_n<T, 256, 4, false>(segs, nseg, st);
-    if (maxS <= 2048) return launch_rank_segs_n<T, 512, 4, false>(segs, nseg, st);
-    if (maxS <= 4096) return launch_rank_segs_n<T, 1024, 4, false>(segs, nseg, st);
-    if (maxS <= 8192) return launch_rank_segs_n<T, 1024, 8, false>(segs, nseg, st);
-    if (maxS <= 16384) return launch_rank_segs_n<T, 1024, 16, sizeof(T) == 8>(segs, nseg, st);
+    if (maxS <= 256) return launch_rank_segs_n<T, 256, 1, 0>(segs, nseg, st);
+    if (maxS <= 512) return launch_rank_segs_n<T, 256, 2, 0>(segs, nseg, st);
+    if (maxS <= 1024) return launch_rank_segs_n<T, 256, 4, 0>(segs, nseg, st);
+    if (maxS <= 2048) return launch_rank_segs_n<T, 512, 4, 0>(segs, nseg, st);
+    if (maxS <= 4096) return launch_rank_segs_n<T, 1024, 4, 0>(segs, nseg, st);
+    if (maxS <= 8192) return launch_rank_segs_n<T, 1024, 8, 0>(segs, nseg, st);
+    if (maxS <= 16384) return launch_rank_segs_n<T, 1024, 16, sizeof(T) == 8 ? 1 : 0>(segs, nseg, st);
     return hipErrorInvalidValue;
 }
 

@@ -22,6 +22,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+VALU_PEAK = 256 * 2.4e9  # MI355X: 256 CUs x one wavefront VALU instruction per clock
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
@@ -98,17 +99,24 @@ def main():
     ex = kr.createSingleThreadExecutor()
     torch.cuda.synchronize()
 
+    # runs fork from and join into torch's current stream, so torch events
+    # between them time each step without adding any synchronisation
+    st = torch.cuda.current_stream()
     for _ in range(args.warmup):
-        kr.runGraphDevice(ex, mod, ptrs, S, T, 0, T)
+        kr.runGraphDevice(ex, mod, ptrs, S, T, 0, T, stream=st.cuda_stream)
     ex.synchronize()
     torch.cuda.synchronize()
+    marks = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        kr.runGraphDevice(ex, mod, ptrs, S, T, 0, T)
+    marks[0].record(st)
+    for k in range(args.steps):
+        kr.runGraphDevice(ex, mod, ptrs, S, T, 0, T, stream=st.cuda_stream)
+        marks[k + 1].record(st)
     ex.synchronize()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     ms = elapsed / args.steps * 1e3
+    step_ms = sorted(marks[k].elapsed_time(marks[k + 1]) for k in range(args.steps))
     cells = len(outputs) * S * T
     value = cells / (ms / 1e3)
 
@@ -181,6 +189,27 @@ def main():
             traffic_note = f"rocprofv3 counters recorded at commit {rec.get('commit')}"
         elif tk:
             traffic_note = "traffic record is of another build of this kernel's library"
+    # VALU instructions of the dominant kernel (rocprofv3 SQ counters,
+    # recorded with the sha256 of the library the kernel lives in)
+    valu_rec, valu_note = None, "no VALU counter record for this kernel"
+    vpath = os.path.join(ROOT, "profiles", f"{args.module}_{S}x{T}_valu.json")
+    if os.path.exists(vpath):
+        with open(vpath) as f:
+            rec = json.load(f)
+        vk = rec.get("kernels", {}).get(dom_name)
+        lib_path = RUNTIME_LIB if dom_key in ("rank", "scale") else catalog.library_path(args.module)
+        want = rec.get("runtime_sha256" if dom_key in ("rank", "scale") else "module_sha256")
+        if vk and want == sha256_file(lib_path):
+            rate = vk["valu_insts"] / (dom_ms / 1e3)
+            valu_rec = {"insts_per_launch": vk["valu_insts"], "alone_us": vk["avg_us"],
+                        "alone_frac_of_valu_peak": vk["valu_frac"],
+                        "in_graph_rate": rate, "in_graph_frac_of_valu_peak": rate / VALU_PEAK,
+                        "peak": VALU_PEAK, "unit": "wavefront VALU instructions/s",
+                        "source": f"rocprofv3 SQ_INSTS_VALU recorded at commit {rec.get('commit')}"}
+            valu_note = None
+        elif vk:
+            valu_note = "VALU record is of another build of this kernel's library"
+    timeline = dag_timeline(kr, mod, ptrs, S, T, ms)
     graph_bytes = (len(inputs) + len(outputs)) * S * T * esz
     graph_gbs = graph_bytes / (ms / 1e3) / 1e9
     by_kind = {}
@@ -195,6 +224,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms,
+        "step_ms": {"median": step_ms[len(step_ms) // 2], "min": step_ms[0], "max": step_ms[-1],
+                    "note": "per step, torch events between the timed runs"},
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -207,7 +238,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": traffic_note,
                      "kernel": f"{dom_name} ({dom['label']})", "launches_per_step": dom["launches"],
-                     "kernel_ms": dom_ms, "total_ms_per_step": dom["ms"], "algorithmic_bytes": dom_bytes},
+                     "kernel_ms": dom_ms, "total_ms_per_step": dom["ms"], "algorithmic_bytes": dom_bytes,
+                     "valu": valu_rec if valu_rec else valu_note},
+        "timeline": timeline,
         "longest_launch": {"kernel": f"stage {lo_i} kernel {lo_j} ({lo_label})", "kernel_ms": lo_ms,
                            "algorithmic_bytes": lo_bytes,
                            "achieved_GBs": lo_bytes / (lo_ms / 1e3) / 1e9,
@@ -264,6 +297,46 @@ def main():
         base["host"] = host_cpu()
         result["cpu_baseline"] = base
     print(json.dumps(result), flush=True)
+
+
+def dag_timeline(kr, mod, ptrs, S, T, wall_ms):
+    """Every kernel of one run timed alone (an executor without lanes runs
+    them one after another, HIP events around each): their sum is the run's
+    serial work, and the longest path of the launch DAG weighted by those
+    times is its critical path -- no schedule finishes sooner."""
+    old = os.environ.get("KUN_STAGE_LANES")
+    os.environ["KUN_STAGE_LANES"] = "0"
+    try:
+        ser = kr.createSingleThreadExecutor()
+    finally:
+        if old is None:
+            del os.environ["KUN_STAGE_LANES"]
+        else:
+            os.environ["KUN_STAGE_LANES"] = old
+    kr.runGraphDevice(ser, mod, ptrs, S, T, 0, T)
+    ser.synchronize()
+    ser.enable_profiling(True)
+    kr.runGraphDevice(ser, mod, ptrs, S, T, 0, T)
+    ser.synchronize()
+    alone = {}
+    for i, _ in enumerate(mod.stages()):
+        for j in range(max(1, len(mod.kernels(i)))):
+            tot, n = ser.kernel_time(mod, i, j)
+            alone[(i, j)] = tot / max(n, 1)
+    ser.enable_profiling(False)
+    finish = {}
+    for line in kr.describeGraphBatch([(mod, S, T, 0, T)]).splitlines():
+        if not line.startswith("dag "):
+            continue
+        head, deps = line.split(":")
+        _, _, node, stage, kern, _ = head.split()
+        start = max((finish[int(d)] for d in deps.split()), default=0.0)
+        finish[int(node)] = start + alone.get((int(stage), int(kern)), 0.0)
+    serial = sum(alone.values())
+    crit = max(finish.values())
+    return {"serial_ms": serial, "critical_path_ms": crit, "dag_nodes": len(finish),
+            "wall_ms": wall_ms, "work_over_wall": serial / wall_ms,
+            "note": "kernels timed alone; critical path = longest launch-DAG path of those times"}
 
 
 # modules whose graph and compile options the reference's library was built

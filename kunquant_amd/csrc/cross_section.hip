@@ -42,6 +42,13 @@ __device__ __forceinline__ size_t cs_at(int layout, long row, long s, long S, lo
     return ((size_t)(s / blk) * rows + row) * blk + s % blk;
 }
 
+// LAY: the view's layout fixed at compile time (-1: read from the view)
+template <int LAY> __device__ __forceinline__ size_t lay_at(const kun_cs::View &v, long row, long s, long S, int blk) {
+    if (LAY == 1) return (size_t)row * S + s;
+    if (LAY == 2) return (size_t)s;
+    return cs_at(v.layout, row, s, S, v.rows, blk);
+}
+
 // A kernel launch through a sink: the arguments converted to the kernel's
 // parameter types, one pointer to each.
 template <typename... P, size_t... I>
@@ -426,6 +433,7 @@ template <typename T, int BLOCK, int ITEMS> struct CountRank {
 
     static __device__ __forceinline__ uint32_t digit(K k, int sh) { return (uint32_t)(k >> sh) & 15u; }
 
+    template <int LIN, int LOUT>
     static __device__ __forceinline__ void row(Smem &sm, const kun_cs::View &in, const kun_cs::View &out, long S,
                                                int blk, long t) {
         if (S <= 0) return; // uniform: nothing to rank
@@ -438,7 +446,7 @@ template <typename T, int BLOCK, int ITEMS> struct CountRank {
 #pragma unroll
         for (int j = 0; j < ITEMS; j++) {
             const long s = w * 64 * ITEMS + j * 64 + lane;
-            xs[j] = src[cs_at(in.layout, rin, s < S ? s : S - 1, S, in.rows, blk)];
+            xs[j] = src[lay_at<LIN>(in, rin, s < S ? s : S - 1, S, blk)];
         }
         K key[ITEMS];
         uint32_t id2[(ITEMS + 1) / 2]; // stock of each key, two 16-bit ids per register
@@ -480,30 +488,25 @@ template <typename T, int BLOCK, int ITEMS> struct CountRank {
         for (int sh = 0; sh < NB; sh += 4) {
             if (digit(diff, sh) == 0u) continue; // uniform over the workgroup
             moved = true;
-            // 1. per-lane counts; nibble j % 8 of pre[j / 8]: the earlier keys
-            // of this lane with key j's digit (at most 15)
+            // 1. per-lane counts; pre[j]: the earlier keys of this lane with
+            // key j's digit
             uint64_t f[NG];
-            uint32_t pre[NG];
+            uint32_t pre[ITEMS];
 #pragma unroll
-            for (int g = 0; g < NG; g++) {
-                f[g] = 0;
-                pre[g] = 0;
-            }
+            for (int g = 0; g < NG; g++) f[g] = 0;
 #pragma unroll
             for (int j = 0; j < ITEMS; j++) {
                 const uint32_t d4 = digit(key[j], sh) * 4u;
                 uint32_t p = 0;
 #pragma unroll
                 for (int g = 0; g < j / 8; g++) p += (uint32_t)(f[g] >> d4) & 15u;
-                p += (uint32_t)(f[j / 8] >> d4) & 15u;
-                pre[j / 8] |= p << (4 * (j % 8));
+                pre[j] = p + ((uint32_t)(f[j / 8] >> d4) & 15u);
                 f[j / 8] += (uint64_t)1 << d4;
             }
-            // the packed prefixes stay packed (eight 4-bit fields in one
-            // register) rather than being recomputed at the scatter from
-            // eight 64-bit partial counts
+            // kept as computed here rather than recomputed at the scatter
+            // from eight 64-bit partial counts
 #pragma unroll
-            for (int g = 0; g < NG; g++) asm volatile("" : "+v"(pre[g]));
+            for (int j = 0; j < ITEMS; j++) asm volatile("" : "+v"(pre[j]));
 #pragma unroll
             for (int k = 0; k < 8; k++) {
                 uint32_t word = 0;
@@ -548,7 +551,7 @@ template <typename T, int BLOCK, int ITEMS> struct CountRank {
             }
 #pragma unroll
             for (int j = 0; j < ITEMS; j++) {
-                const uint32_t at = pos[j] + ((pre[j / 8] >> (4 * (j % 8))) & 15u);
+                const uint32_t at = pos[j] + pre[j];
                 sm.key[at] = key[j];
                 sm.idx[at] = (uint16_t)(id2[j / 2] >> (16 * (j % 2)));
             }
@@ -570,12 +573,13 @@ template <typename T, int BLOCK, int ITEMS> struct CountRank {
             for (int j = 0; j < ITEMS; j++) sm.key[tid * ITEMS + j] = key[j];
             __syncthreads();
         }
-        ties(sm, key, id2, n, dst, out, S, blk, t);
+        ties<LOUT>(sm, key, id2, n, dst, out, S, blk, t);
     }
 
     static __device__ __forceinline__ T rank_of(int a, int b, int n) { return group_rank<T>(a, b, n); }
 
     // sorted keys at positions P = tid*ITEMS + j (registers and sm.key)
+    template <int LOUT>
     static __device__ __forceinline__ void ties(Smem &sm, const K (&key)[ITEMS], const uint32_t (&id2)[(ITEMS + 1) / 2],
                                                 int n, T *dst, const kun_cs::View &out, long S, int blk, long t) {
         const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -622,7 +626,7 @@ template <typename T, int BLOCK, int ITEMS> struct CountRank {
             if (id < S) res[id] = r[j];
         }
         __syncthreads();
-        for (long s = tid; s < S; s += BLOCK) dst[cs_at(out.layout, t + out.row0, s, S, out.rows, blk)] = res[s];
+        for (long s = tid; s < S; s += BLOCK) dst[lay_at<LOUT>(out, t + out.row0, s, S, blk)] = res[s];
     }
 };
 
@@ -633,7 +637,10 @@ __device__ __forceinline__ void rank_row_k(const kun_cs::View &in, const kun_cs:
     if constexpr (KIND == 0) {
         using R = CountRank<T, BLOCK, ITEMS>;
         __shared__ typename R::Smem sm;
-        R::row(sm, in, out, S, blk, t);
+        // rows of TS buffers (every batch run's temporaries) with their
+        // addressing fixed at compile time; other layouts read the view's
+        if (in.layout == 1 && out.layout == 1) R::template row<1, 1>(sm, in, out, S, blk, t);
+        else R::template row<-1, -1>(sm, in, out, S, blk, t);
     } else {
         using R = KeysRank<T, BLOCK, ITEMS>;
         __shared__ typename R::Smem sm;
@@ -669,12 +676,6 @@ constexpr int SCALE_BLOCK = 256;
 constexpr int SCALE_ROWS = 8;
 constexpr int SCALE_SUM_ROWS = 4;
 constexpr int SCALE_TILE = 1024;
-
-template <int LAY> __device__ __forceinline__ size_t lay_at(const kun_cs::View &v, long row, long s, long S, int blk) {
-    if (LAY == 1) return (size_t)row * S + s;
-    if (LAY == 2) return (size_t)s;
-    return cs_at(v.layout, row, s, S, v.rows, blk);
-}
 
 // sums != null: store the row sums there and stop
 template <typename T, int LIN, int LOUT, int ROWS>

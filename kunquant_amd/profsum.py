@@ -77,14 +77,15 @@ def trace(args):
     for n, v in sorted(by.items(), key=lambda kv: -sum(kv[1]))[:args.top]:
         out.append(f"| {n} | {len(v)} | {sum(v) / 1e6:.2f} | {sum(v) / len(v) / 1e3:.1f} | "
                    f"{100 * sum(v) / total:.1f} |")
-    if args.launches:
+    if args.launches or args.runs:
         # runs submitted back to back: the library's kernels in dispatch
         # order, `launches` per run
         own = sorted(((int(r["Dispatch_Id"]), int(r["Start_Timestamp"]), int(r["End_Timestamp"]),
                        short_name(r["Kernel_Name"])) for r in rows), key=lambda x: x[0])
         own = [x[1:] for x in own if kernel_class(x[3]) != "other"]
-        runs = [sorted(own[i:i + args.launches]) for i in range(0, len(own) - args.launches + 1, args.launches)]
-        how = f"runs of {args.launches} launches in dispatch order"
+        n = args.launches or len(own) // args.runs
+        runs = [sorted(own[i:i + n]) for i in range(0, len(own) - n + 1, n)]
+        how = f"runs of {n} launches in dispatch order"
     else:
         runs, cur, end = [], [], 0
         for k in ks:
@@ -164,7 +165,7 @@ def valu(args):
                    f"{r['valu_rate'] / 1e9:.1f} | {r['valu_frac']:.3f} | {act} | {lds} |")
     print("\n".join(out))
     if args.json:
-        rec = {"command": args.command, "peak_valu_insts_per_s": VALU_PEAK, "commit": _commit(),
+        rec = {"command": args.command, "peak_valu_insts_per_s": VALU_PEAK, "commit": args.commit or _commit(),
                "runtime_sha256": _sha(_runtime_lib()), "kernels": {r["kernel"]: r for r in rows}}
         if args.module:
             rec["module_sha256"] = _sha(_module_lib(args.module))
@@ -227,7 +228,7 @@ def traffic(args):
     divs = [k for k in kern if k.startswith("scale_div<")]
     if sums and divs:  # one Scale stage = row sums + division, as bench.py groups them
         kern["scale_sums + scale_div"] = {x: kern[sums[0]][x] + kern[divs[0]][x] for x in ("fetch_bytes", "write_bytes")}
-    rec = {"module": args.module, "stocks": args.stocks, "days": args.days, "commit": _commit(),
+    rec = {"module": args.module, "stocks": args.stocks, "days": args.days, "commit": args.commit or _commit(),
            "module_sha256": _sha(_module_lib(args.module)), "runtime_sha256": _sha(_runtime_lib()),
            "scale_fetch": sf, "scale_write": sw, "command": args.command,
            "method": "FETCH_SIZE and WRITE_SIZE (KiB per dispatch) x 1024 x (1 GiB / the counter's reading "
@@ -243,10 +244,12 @@ def main(argv=None):
     t = sub.add_parser("trace")
     t.add_argument("csv")
     t.add_argument("--launches", type=int, default=0, help="kernel launches per run (runs back to back)")
+    t.add_argument("--runs", type=int, default=0, help="number of runs the library's launches divide into")
     v = sub.add_parser("valu")
     v.add_argument("csv")
     v.add_argument("--json")
     v.add_argument("--module")
+    v.add_argument("--commit", help="commit the libraries were built from (default: git HEAD)")
     for p in (t, v):
         p.add_argument("--title", default="")
         p.add_argument("--command", default="")
@@ -259,6 +262,7 @@ def main(argv=None):
     r.add_argument("--days", type=int, required=True)
     r.add_argument("--json", required=True)
     r.add_argument("--command", default="")
+    r.add_argument("--commit", help="commit the libraries were built from (default: git HEAD)")
     a = ap.parse_args(argv)
     {"trace": trace, "valu": valu, "traffic": traffic}[a.cmd](a)
 

@@ -21,9 +21,12 @@ Stream stage kernel (STREAM modules): the same op lowering for one row;
 rings and carried state live in HBM as [slot][S] arrays owned by the stream
 context (the reference's StreamBuffer / StateBuffer, Runtime.cpp:384-546).
 Cross-sectional stages (Rank / Scale) are runtime library kernels."""
+import math
 import os
 import re
 from typing import Dict, List, Optional, Tuple
+
+import numpy as np
 
 from ..Driver import BUF_INPUT, BUF_TEMP, ModuleIR, STAGE_GENERIC, STAGE_TS
 from ..Op import ConstantOp, ForeachBackWindow, Input, OpBase, Output, ReductionOp
@@ -265,6 +268,11 @@ class _Lowering:
             c = c_literal(op.attrs["value"], T)
             x, y = (c, ins[0]) if op.attrs.get("swap", False) else (ins[0], c)
             expr = f"({x} {_CONST_BIN[name]} {y})"
+            cf = float(np.float32(op.attrs["value"]))
+            if name == "DivConst" and T == "float" and not op.attrs.get("swap", False) \
+                    and math.isfinite(cf) and cf != 0.0:
+                # float division by a constant as a double product (exact, see kun::div_const)
+                expr = f"kun::div_const({ins[0]}, {float.hex(1.0 / cf)})"
             if name.endswith("ThanConst"):
                 expr = f"kun::mask<{T}>{expr}"
             L.append(f"const {T} {out} = {expr};")

@@ -57,6 +57,14 @@ template <typename T> __device__ __forceinline__ T vabs(T a) {
     using U = typename Bits<T>::U;
     return from_bits<T>(to_bits(a) & ~(U(1) << (Bits<T>::NBITS - 1)));
 }
+// x / c for a finite, nonzero float constant c, rounded exactly as IEEE
+// float division: rc = RN(1/c) in double, and the double product x * rc is
+// within 2^-28 ulp of x / c, while x / c is never nearer than ulp / 2^25 to
+// a midpoint between floats (the odd part of c's significand has at most
+// 24 bits) -- so rounding the product to float rounds x / c. Three VALU
+// operations instead of the division sequence; checked for every float x
+// and a set of constants by tests/native/div_const_check.c.
+__device__ __forceinline__ float div_const(float x, double rc) { return (float)((double)x * rc); }
 __device__ __forceinline__ float vsqrt(float a) { return __builtin_sqrtf(a); }
 __device__ __forceinline__ double vsqrt(double a) { return __builtin_sqrt(a); }
 __device__ __forceinline__ float fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }

@@ -49,6 +49,11 @@ def main():
     ap.add_argument("--seconds", type=float, default=3.0, help="--workload serve: length of the arrival trace")
     ap.add_argument("--state-budget-gb", type=float, default=64.0,
                     help="--workload serve: HBM for session rows and state (admission limit)")
+    ap.add_argument("--inflight", type=int, default=4, help="--workload serve: cycles in flight")
+    ap.add_argument("--max-computing", type=int, default=3,
+                    help="--workload serve: cycles in flight that may still be computing when another starts")
+    ap.add_argument("--readback", choices=("all", "none"), default="all",
+                    help="--workload serve: copy every output row of each cycle to host memory inside the cycle")
     ap.add_argument("--jobs", type=int, default=64, help="jobs of --workload jobs")
     ap.add_argument("--no-host-baseline", action="store_true",
                     help="--workload jobs: skip the sequential host-buffer runGraph pass")
@@ -569,8 +574,8 @@ def run_serve(args):
     bps = kr.StreamPool.bytes_per_stock(mod)
     cap = int(args.state_budget_gb * 1e9 // bps)
     ex = kr.createSingleThreadExecutor()
-    pool = kr.StreamPool(ex, mod, cap, max_sessions=args.sessions, max_inflight=2)
-    server = TickServer(pool, max_inflight=2)
+    pool = kr.StreamPool(ex, mod, cap, max_sessions=args.sessions, max_inflight=args.inflight)
+    server = TickServer(pool, max_inflight=args.inflight, max_computing=args.max_computing)
     for j, S in enumerate(sizes):
         server.open(j, S)
     live = sorted(server.sid)
@@ -594,6 +599,34 @@ def run_serve(args):
     print(f"serve: warm-up {sat_s:.2f} s, {sat_cycles} cycles", file=sys.stderr, flush=True)
     server.latencies.clear()
     server.cycles = server.ticks_in_cycles = 0
+
+    # results to host inside every cycle (latency then includes the readback)
+    outs_rb = mod.getOutputNames() if args.readback == "all" else []
+    pool.set_readback(outs_rb)
+    # per-phase breakdown of full cycles (every session ticks once)
+    from kunquant_amd.codegen.hip import stream_phases
+    phases = stream_phases(catalog.module_ir("serve_f32"))
+    pool.profile(True)
+    prof = []
+    for t in range(4):
+        c = pool.submit([rows[j][t % NR] for j in live])
+        pool.done(c, wait=True)
+        prof.append(pool.phase_times(c))
+    pool.profile(False)
+    pt = np.median(np.array(prof[1:]), axis=0)
+    kinds = ["time-series" if k == 0 else "cross-sectional" for k, _ in phases]
+    ph_rows = [{"phase": i, "kind": kinds[i], "stages": len(phases[i][1]), "ms": float(pt[i + 1])}
+               for i in range(len(phases))]
+    breakdown = {"input_staging_ms": float(pt[0]), "readback_ms": float(pt[-1]),
+                 "time_series_phases_ms": float(sum(r["ms"] for r in ph_rows if r["kind"] == "time-series")),
+                 "cross_sectional_phases_ms": float(sum(r["ms"] for r in ph_rows if r["kind"] != "time-series")),
+                 "cycle_ms": float(pt.sum()), "phases": len(ph_rows),
+                 "slowest_phases": sorted(ph_rows, key=lambda r: -r["ms"])[:8],
+                 "what": f"one cycle of all {len(live)} sessions, events between phases, median of 3"}
+    print(f"serve: full cycle {breakdown['cycle_ms']:.2f} ms: staging {breakdown['input_staging_ms']:.2f}, "
+          f"time-series {breakdown['time_series_phases_ms']:.2f}, cross-sectional "
+          f"{breakdown['cross_sectional_phases_ms']:.2f}, readback {breakdown['readback_ms']:.2f}",
+          file=sys.stderr, flush=True)
 
     # open loop: Poisson arrivals over all sessions
     def open_loop(rate):
@@ -627,6 +660,12 @@ def run_serve(args):
     sweep = [open_loop(float(x))[0] for x in args.rate_sweep.split(",") if x]
     main_r, lat = open_loop(args.rate)
     p50, p99, achieved = main_r["p50_ms"], main_r["p99_ms"], main_r["achieved_ticks_per_s"]
+    hbm_only = None
+    if outs_rb:
+        # the same offered rate with results left in HBM (no readback)
+        pool.set_readback([])
+        hbm_only = open_loop(args.rate)[0]
+        hbm_only["latency"] = "arrival to the tick's results complete in HBM"
 
     # plain per-session path: one stream context per session, one tick at a time
     plain = []
@@ -663,7 +702,14 @@ def run_serve(args):
         "serving": {**main_r,
                     "rate_sweep": sweep,
                     "saturated_ticks_per_s": 60 * len(live) / sat_s, "saturated_ms_per_cycle": sat_s / sat_cycles * 1e3,
-                    "latency": "arrival (scheduled time) to the tick's results complete in HBM",
+                    "latency": ("arrival (scheduled time) to the tick's results in host memory (every output "
+                                "row copied to pinned host memory inside the cycle)" if outs_rb else
+                                "arrival (scheduled time) to the tick's results complete in HBM"),
+                    "readback_bytes_per_tick": len(outs_rb) * float(np.mean([sizes[j] for j in live])) * 4,
+                    "cycle_breakdown": breakdown,
+                    "same_rate_results_in_hbm": hbm_only,
+                    "cycle_share": server.cycle_share, "cycles_in_flight": args.inflight,
+                    "max_computing": server.max_computing,
                     "plain_per_session": {"p50_ms": float(np.percentile(plain, 50)),
                                           "p99_ms": float(np.percentile(plain, 99)),
                                           "ticks_per_s": 1e3 / float(plain.mean()),

@@ -254,6 +254,26 @@ KUN_API int kunStreamPoolRead(KunStreamPoolHandle pool, int session, size_t buff
 KUN_API const void *kunStreamPoolRowDevice(KunStreamPoolHandle pool, int session, size_t buffer_handle);
 /* ticks submitted for the session so far; -1 on error */
 KUN_API long kunStreamPoolTicks(KunStreamPoolHandle pool, int session);
+/* Results to host inside each cycle: every later cycle ends by copying the
+ * rows of these buffers (handles as kunQueryBufferHandle's) of its sessions
+ * to pinned host memory, so a finished cycle's results are in host memory
+ * when kunStreamPoolDone reports it. nbuffers = 0 turns it off. Waits for
+ * cycles in flight. */
+KUN_API int kunStreamPoolSetReadback(KunStreamPoolHandle pool, int nbuffers, const size_t *buffer_handles);
+/* 1: the cycle's phases are done (its sessions' rows are final in HBM; the
+ * copy to host may still run), 0: running (wait = 0 only), -1: error */
+KUN_API int kunStreamPoolComputed(KunStreamPoolHandle pool, unsigned long long cycle, int wait);
+/* Host address of a session's row of a readback buffer from cycle `cycle`
+ * (waits for the cycle). Valid until max_inflight more cycles have been
+ * submitted after it (slots are reused oldest first); NULL on error. */
+KUN_API const void *kunStreamPoolResult(KunStreamPoolHandle pool, unsigned long long cycle, int session,
+                                        size_t buffer_handle);
+/* Per-phase timing of later cycles (enable = 1): kunStreamPoolPhaseTimes
+ * fills ms[0] = input staging (copy + scatter), ms[1 .. nphase] = the
+ * module's phases in launch order, ms[nphase + 1] = readback (0 without),
+ * waiting for the cycle; returns nphase + 2, or -1 on error. */
+KUN_API int kunStreamPoolProfile(KunStreamPoolHandle pool, int enable);
+KUN_API int kunStreamPoolPhaseTimes(KunStreamPoolHandle pool, unsigned long long cycle, float *ms, int n);
 /* A session resumed from a serialised stream state in the reference's byte
  * layout (kunStreamSerializeStates of either library, same module and
  * stocks). Return codes as kunStreamPoolAdmit; a bad blob is error 1 with

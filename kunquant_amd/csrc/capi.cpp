@@ -503,6 +503,33 @@ KUN_API long kunStreamPoolTicks(KunStreamPoolHandle pool, int session) {
     return t;
 }
 
+KUN_API int kunStreamPoolSetReadback(KunStreamPoolHandle pool, int nbuffers, const size_t *buffer_handles) {
+    return guarded([&] { pool_set_readback((StreamPool *)pool, nbuffers, buffer_handles); });
+}
+
+KUN_API const void *kunStreamPoolResult(KunStreamPoolHandle pool, unsigned long long cycle, int session,
+                                        size_t buffer_handle) {
+    const void *r = nullptr;
+    guarded([&] { r = pool_result((StreamPool *)pool, cycle, session, buffer_handle); });
+    return r;
+}
+
+KUN_API int kunStreamPoolComputed(KunStreamPoolHandle pool, unsigned long long cycle, int wait) {
+    int r = -1;
+    if (guarded([&] { r = pool_computed((StreamPool *)pool, cycle, wait != 0); })) return -1;
+    return r;
+}
+
+KUN_API int kunStreamPoolProfile(KunStreamPoolHandle pool, int enable) {
+    return guarded([&] { pool_profile((StreamPool *)pool, enable != 0); });
+}
+
+KUN_API int kunStreamPoolPhaseTimes(KunStreamPoolHandle pool, unsigned long long cycle, float *ms, int n) {
+    int k = -1;
+    if (guarded([&] { k = pool_phase_times((StreamPool *)pool, cycle, ms, n); })) return -1;
+    return k;
+}
+
 KUN_API int kunStreamPoolAdmitState(KunStreamPoolHandle pool, size_t num_stocks, const char *state, size_t state_len,
                                     int *session) {
     if (!state || !session) {

@@ -256,6 +256,11 @@ int pool_done(StreamPool *p, unsigned long long cycle, bool wait);
 const void *pool_row_device(StreamPool *p, int sid, size_t b);
 void pool_read(StreamPool *p, int sid, size_t b, void *host);
 long pool_ticks(StreamPool *p, int sid);
+void pool_set_readback(StreamPool *p, int n, const size_t *bufs);
+const void *pool_result(StreamPool *p, unsigned long long cycle, int sid, size_t b);
+void pool_profile(StreamPool *p, bool on);
+int pool_computed(StreamPool *p, unsigned long long cycle, bool wait);
+int pool_phase_times(StreamPool *p, unsigned long long cycle, float *ms, int n);
 int pool_admit_state(StreamPool *p, size_t S, const char *state, size_t len);
 std::string pool_serialize(StreamPool *p, int sid);
 

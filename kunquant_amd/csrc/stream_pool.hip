@@ -21,6 +21,14 @@
 // disjoint sessions are on the device at once (a session belongs to at most
 // one cycle in flight), so ticks that arrive while a cycle runs start
 // without waiting for it.
+//
+// Readback: with a set of readback buffers (pool_set_readback), a cycle ends
+// by gathering those rows of its sessions into one packed device block
+// ([buffer][lanes of the cycle]) and copying it to the cycle slot's pinned
+// host block, so that a finished cycle's results are already in host memory
+// (pool_result points into that block until the slot's next cycle).
+// Profiling (pool_profile) records events between the input staging, the
+// phases and the readback of each cycle (pool_phase_times).
 #include <hip/hip_runtime.h>
 #include <string.h>
 
@@ -56,6 +64,22 @@ template <typename T> __global__ __launch_bounds__(64) void pool_scatter(Scatter
     if (s >= e.s1) return;
     const long k = e.p0 + threadIdx.x;
     for (int q = 0; q < a.nin; q++) ((T *)a.dst[q])[s] = ((const T *)a.packed)[(size_t)q * a.total + k];
+}
+
+struct GatherArgs {
+    const void *const *src; // pool rows of the readback buffers
+    void *packed;           // [buffer][total] results of the cycle's sessions
+    long total;
+    const kun::PoolBlock *blocks;
+    int nq;
+};
+
+template <typename T> __global__ __launch_bounds__(64) void pool_gather(GatherArgs a) {
+    const kun::PoolBlock e = a.blocks[blockIdx.x];
+    const long s = (long)e.s0 + threadIdx.x;
+    if (s >= e.s1) return;
+    const long k = e.p0 + threadIdx.x;
+    for (int q = 0; q < a.nq; q++) ((T *)a.packed)[(size_t)q * a.total + k] = ((const T *)a.src[q])[s];
 }
 
 // NaN into [s0, s0 + n) of `nrow` rows of `stride` values
@@ -152,11 +176,19 @@ struct PoolSession {
 struct PoolSlot {
     hipStream_t st = nullptr;
     hipEvent_t done = nullptr;
+    hipEvent_t computed = nullptr; // phases (and the readback gather) finished
     char *host = nullptr; // pinned staging
     char *dev = nullptr;
     bool busy = false;
     unsigned long long cycle = 0;
     std::vector<int> sessions;
+    // readback: the cycle's results, [buffer][res_total] (pinned, device)
+    char *res_host = nullptr, *res_dev = nullptr;
+    size_t res_total = 0;
+    std::map<int, size_t> res_off; // session -> first lane in the block
+    // profiling: events at the cycle's boundaries (timing enabled)
+    std::vector<hipEvent_t> ph_ev;
+    bool profiled = false;
 };
 
 struct StreamPool {
@@ -177,6 +209,9 @@ struct StreamPool {
     unsigned long long next_cycle = 1;
     char *scratch = nullptr;              // admission block table
     std::unique_ptr<CopyWorkers> copiers;
+    std::vector<int> rb_bufs;             // readback buffer handles
+    const void **d_rb_src = nullptr;      // their pool rows (device array)
+    bool profile = false;
     ~StreamPool();
     void *row(int b) const { return rows + (size_t)b * cap * esz; }
 };
@@ -188,9 +223,14 @@ StreamPool::~StreamPool() {
         if (sl.st) hipStreamSynchronize(sl.st);
         if (sl.st) hipStreamDestroy(sl.st);
         if (sl.done) hipEventDestroy(sl.done);
+        if (sl.computed) hipEventDestroy(sl.computed);
         if (sl.host) hipHostFree(sl.host);
         if (sl.dev) hipFree(sl.dev);
+        if (sl.res_host) hipHostFree(sl.res_host);
+        if (sl.res_dev) hipFree(sl.res_dev);
+        for (auto e : sl.ph_ev) hipEventDestroy(e);
     }
+    if (d_rb_src) hipFree(d_rb_src);
     hipStreamSynchronize(ex->stream);
     if (rows) hipFree(rows);
     for (void *p : state)
@@ -275,6 +315,7 @@ StreamPool *pool_create(std::shared_ptr<Executor> ex, const KunModuleDesc *m, si
     for (auto &sl : p->slots) {
         HIPCHECK(hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking));
         HIPCHECK(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+        HIPCHECK(hipEventCreateWithFlags(&sl.computed, hipEventDisableTiming));
         HIPCHECK(hipHostMalloc((void **)&sl.host, p->staging, hipHostMallocDefault));
         HIPCHECK(hipMalloc((void **)&sl.dev, p->staging));
     }
@@ -389,11 +430,13 @@ void pool_release(StreamPool *p, int sid) {
 unsigned long long pool_submit(StreamPool *p, int n, const int *sessions, const void *const *inputs) {
     const KunModuleDesc *m = p->m;
     if (n < 1) throw Error("a cycle needs at least one session");
-    // a free slot: the oldest busy one is waited for if none is free
+    // the free slot that ran the oldest cycle (so a cycle's readback block
+    // outlives the next max_inflight - 1 submissions); the oldest busy one
+    // is waited for if none is free
     PoolSlot *sl = nullptr;
     for (auto &x : p->slots) {
         if (x.busy && hipEventQuery(x.done) == hipSuccess) retire(p, x);
-        if (!x.busy && !sl) sl = &x;
+        if (!x.busy && (!sl || x.cycle < sl->cycle)) sl = &x;
     }
     if (!sl) {
         sl = &p->slots[0];
@@ -467,6 +510,17 @@ unsigned long long pool_submit(StreamPool *p, int n, const int *sessions, const 
     }
     hipStream_t st = sl->st;
     HIPCHECK(hipSetDevice(p->ex->device));
+    sl->profiled = p->profile;
+    if (p->profile)
+        while (sl->ph_ev.size() < (size_t)m->nphase + 3) {
+            hipEvent_t e;
+            HIPCHECK(hipEventCreate(&e));
+            sl->ph_ev.push_back(e);
+        }
+    auto mark = [&](int k) {
+        if (p->profile) HIPCHECK(hipEventRecord(sl->ph_ev[k], st));
+    };
+    mark(0);
     HIPCHECK(hipMemcpyAsync(sl->dev, h, used, hipMemcpyHostToDevice, st));
     const kun::PoolBlock *dblk = (const kun::PoolBlock *)(sl->dev + off_blocks);
     const kun_cs::Seg *dseg = (const kun_cs::Seg *)(sl->dev + off_segs);
@@ -482,6 +536,7 @@ unsigned long long pool_submit(StreamPool *p, int n, const int *sessions, const 
     else
         hipLaunchKernelGGL(pool_scatter<float>, dim3((unsigned)nblocks), dim3(64), 0, st, sa);
     HIPCHECK(hipGetLastError());
+    mark(1);
     kun::PoolArgs pa{p->d_args, dblk};
     void *args[] = {&pa};
     size_t so = 0;
@@ -489,6 +544,7 @@ unsigned long long pool_submit(StreamPool *p, int n, const int *sessions, const 
         const KunPhaseDesc &pd = m->phases[ph];
         if (pd.kind == KUN_PHASE_TS) {
             HIPCHECK(hipLaunchKernel(pd.kernel, dim3((unsigned)nblocks), dim3(64), args, 0, st));
+            mark(ph + 2);
             continue;
         }
         const int nr = p->ncs_rank[ph] * n, ns = p->ncs_scale[ph] * n;
@@ -496,7 +552,26 @@ unsigned long long pool_submit(StreamPool *p, int n, const int *sessions, const 
         if (e == hipSuccess) e = kun_cs::launch_scale_segs(m->dtype, dseg + so + nr, ns, st);
         if (e != hipSuccess) throw Error(std::string("cross-sectional kernel: ") + hipGetErrorString(e));
         so += (size_t)nr + ns;
+        mark(ph + 2);
     }
+    sl->res_off.clear();
+    sl->res_total = 0;
+    if (!p->rb_bufs.empty()) {
+        GatherArgs ga{p->d_rb_src, sl->res_dev, (long)total, dblk, (int)p->rb_bufs.size()};
+        if (p->esz == 8)
+            hipLaunchKernelGGL(pool_gather<double>, dim3((unsigned)nblocks), dim3(64), 0, st, ga);
+        else
+            hipLaunchKernelGGL(pool_gather<float>, dim3((unsigned)nblocks), dim3(64), 0, st, ga);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipEventRecord(sl->computed, st));
+        HIPCHECK(hipMemcpyAsync(sl->res_host, sl->res_dev, p->rb_bufs.size() * total * p->esz,
+                                hipMemcpyDeviceToHost, st));
+        sl->res_total = total;
+        for (int j = 0; j < n; j++) sl->res_off[sessions[j]] = offs[j];
+    } else {
+        HIPCHECK(hipEventRecord(sl->computed, st));
+    }
+    mark(m->nphase + 2);
     HIPCHECK(hipEventRecord(sl->done, st));
     sl->busy = true;
     sl->cycle = p->next_cycle++;
@@ -526,6 +601,18 @@ int pool_done(StreamPool *p, unsigned long long c, bool wait) {
     return 1;
 }
 
+// 1 when cycle `c` has finished its phases (its readback may still run)
+int pool_computed(StreamPool *p, unsigned long long c, bool wait) {
+    for (auto &sl : p->slots)
+        if (sl.busy && sl.cycle == c) {
+            hipError_t e = wait ? hipEventSynchronize(sl.computed) : hipEventQuery(sl.computed);
+            if (e == hipErrorNotReady) return 0;
+            HIPCHECK(e);
+            return 1;
+        }
+    return 1;
+}
+
 const void *pool_row_device(StreamPool *p, int sid, size_t b) {
     const PoolSession &ss = live_session(p, sid);
     if (b >= (size_t)p->m->nbuf) throw Error("bad buffer handle");
@@ -541,6 +628,63 @@ void pool_read(StreamPool *p, int sid, size_t b, void *host) {
 }
 
 long pool_ticks(StreamPool *p, int sid) { return live_session(p, sid).tick; }
+
+void pool_set_readback(StreamPool *p, int n, const size_t *bufs) {
+    if (n < 0 || (n > 0 && !bufs)) throw Error("bad readback buffer list");
+    for (int i = 0; i < n; i++)
+        if (bufs[i] >= (size_t)p->m->nbuf) throw Error("bad buffer handle");
+    HIPCHECK(hipSetDevice(p->ex->device));
+    for (auto &sl : p->slots) {
+        wait_slot(p, sl);
+        if (sl.res_host) HIPCHECK(hipHostFree(sl.res_host));
+        if (sl.res_dev) HIPCHECK(hipFree(sl.res_dev));
+        sl.res_host = sl.res_dev = nullptr;
+        sl.res_off.clear();
+        sl.res_total = 0;
+    }
+    if (p->d_rb_src) HIPCHECK(hipFree(p->d_rb_src));
+    p->d_rb_src = nullptr;
+    p->rb_bufs.assign(bufs, bufs + n);
+    if (!n) return;
+    std::vector<const void *> src(n);
+    for (int i = 0; i < n; i++) src[i] = p->row((int)bufs[i]);
+    HIPCHECK(hipMalloc((void **)&p->d_rb_src, n * sizeof(void *)));
+    HIPCHECK(hipMemcpy(p->d_rb_src, src.data(), n * sizeof(void *), hipMemcpyHostToDevice));
+    const size_t bytes = (size_t)n * p->cap * p->esz;
+    for (auto &sl : p->slots) {
+        HIPCHECK(hipHostMalloc((void **)&sl.res_host, bytes, hipHostMallocDefault));
+        HIPCHECK(hipMalloc((void **)&sl.res_dev, bytes));
+    }
+}
+
+static PoolSlot &cycle_slot(StreamPool *p, unsigned long long c) {
+    for (auto &sl : p->slots)
+        if (sl.cycle == c && c) return sl;
+    throw Error("cycle " + std::to_string(c) + " is not the last cycle of any slot (its results were replaced)");
+}
+
+const void *pool_result(StreamPool *p, unsigned long long c, int sid, size_t b) {
+    PoolSlot &sl = cycle_slot(p, c);
+    size_t q = 0;
+    while (q < p->rb_bufs.size() && p->rb_bufs[q] != b) q++;
+    if (q == p->rb_bufs.size()) throw Error("buffer is not a readback buffer of the pool");
+    auto it = sl.res_off.find(sid);
+    if (it == sl.res_off.end()) throw Error("session " + std::to_string(sid) + " was not in cycle " + std::to_string(c));
+    wait_slot(p, sl);
+    return sl.res_host + ((size_t)q * sl.res_total + it->second) * p->esz;
+}
+
+void pool_profile(StreamPool *p, bool on) { p->profile = on; }
+
+int pool_phase_times(StreamPool *p, unsigned long long c, float *ms, int n) {
+    PoolSlot &sl = cycle_slot(p, c);
+    if (!sl.profiled) throw Error("cycle " + std::to_string(c) + " was not profiled");
+    const int k = p->m->nphase + 2;
+    if (n < k) throw Error("phase time array needs " + std::to_string(k) + " entries");
+    wait_slot(p, sl);
+    for (int i = 0; i < k; i++) HIPCHECK(hipEventElapsedTime(&ms[i], sl.ph_ev[i], sl.ph_ev[i + 1]));
+    return k;
+}
 
 // One session's rows and stage states as a stream context of S stocks
 // would hold them (WireHost), copied out of / into the pool's [..][cap]

@@ -90,6 +90,11 @@ def _load():
     sig("kunStreamPoolRead", i, vp, i, sz, vp)
     sig("kunStreamPoolRowDevice", vp, vp, i, sz)
     sig("kunStreamPoolTicks", ctypes.c_long, vp, i)
+    sig("kunStreamPoolSetReadback", i, vp, i, ctypes.POINTER(sz))
+    sig("kunStreamPoolResult", vp, vp, ctypes.c_ulonglong, i, sz)
+    sig("kunStreamPoolProfile", i, vp, i)
+    sig("kunStreamPoolComputed", i, vp, ctypes.c_ulonglong, i)
+    sig("kunStreamPoolPhaseTimes", i, vp, ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_float), i)
     sig("kunStreamPoolAdmitState", i, vp, sz, ctypes.c_char_p, sz, ctypes.POINTER(i))
     sig("kunStreamPoolSerializeStates", i, vp, i, ctypes.c_char_p, ctypes.POINTER(sz))
     ip = ctypes.POINTER(ctypes.c_int)
@@ -779,6 +784,49 @@ class StreamPool:
 
     def ticks(self, sid: int) -> int:
         return _load().kunStreamPoolTicks(self._h, sid)
+
+    def set_readback(self, names):
+        """Buffers whose rows every later cycle copies to pinned host memory
+        before it reports done (results to host inside the cycle); [] turns
+        it off. Waits for cycles in flight."""
+        hs = (ctypes.c_size_t * max(1, len(names)))(*[self._handle[n] for n in names])
+        if _load().kunStreamPoolSetReadback(self._h, len(names), hs):
+            raise RuntimeError(_err())
+        self._readback = list(names)
+
+    def result(self, cycle: int, sid: int, name: str, copy: bool = True) -> np.ndarray:
+        """A session's row of a readback buffer as cycle `cycle` left it, from
+        the cycle's pinned host block (waits for the cycle); available until
+        max_inflight more cycles have been submitted. copy=False gives a view
+        that a later cycle overwrites."""
+        p = _load().kunStreamPoolResult(self._h, cycle, sid, self._handle[name])
+        if not p:
+            raise RuntimeError(_err())
+        ct = ctypes.c_float if self._mod.dtype == np.float32 else ctypes.c_double
+        v = np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ct)), shape=(self._stocks[sid],))
+        return v.copy() if copy else v
+
+    def computed(self, cycle: int, wait: bool = False) -> bool:
+        """True once the cycle's phases are done (results final in HBM; the
+        copy to host of readback buffers may still run)."""
+        r = _load().kunStreamPoolComputed(self._h, cycle, 1 if wait else 0)
+        if r < 0:
+            raise RuntimeError(_err())
+        return r == 1
+
+    def profile(self, on: bool = True):
+        """Record per-phase events in later cycles (see phase_times)."""
+        if _load().kunStreamPoolProfile(self._h, 1 if on else 0):
+            raise RuntimeError(_err())
+
+    def phase_times(self, cycle: int) -> np.ndarray:
+        """ms of a profiled cycle: [input staging, phase 0 .. phase n-1,
+        readback] (waits for the cycle)."""
+        buf = (ctypes.c_float * 4096)()
+        k = _load().kunStreamPoolPhaseTimes(self._h, cycle, buf, 4096)
+        if k < 0:
+            raise RuntimeError(_err())
+        return np.array(buf[:k], dtype=np.float64)
 
     def serializeStates(self, sid: int) -> bytes:
         """The session's state in the reference's stream-state byte layout."""

@@ -7,7 +7,17 @@ state footprint). Ticks queue per session; every dispatch ticks each session
 that has a queued tick and no tick in flight, in one pool cycle, while up to
 `max_inflight` cycles run on the device: ticks that arrive while a cycle runs
 are picked up by the next one without waiting for it. Ticks of one session
-run in arrival order, one per cycle."""
+run in arrival order, one per cycle.
+
+Under load a cycle takes at most `cycle_share` of the admitted sessions
+(default: 1 / max_inflight of them, in order of readiness), so that cycles
+overlap on the device, and at most `max_computing` of the cycles in flight
+may still be running their phases when another starts (default: all of
+them): with results copied to host inside the cycle, a limit below
+max_inflight keeps the copies of some cycles running while others compute,
+instead of cycles that start together computing together and copying
+together."""
+import math
 import time
 from collections import deque
 from typing import Callable, Deque, Dict, Hashable, List, Optional, Tuple
@@ -17,9 +27,12 @@ from .runner import KunRunner as kr
 
 class TickServer:
     def __init__(self, pool: "kr.StreamPool", max_inflight: int = 2, clock: Callable[[], float] = time.perf_counter,
-                 on_done: Optional[Callable[[Hashable, int, float], None]] = None):
+                 on_done: Optional[Callable[[Hashable, int, float], None]] = None,
+                 cycle_share: Optional[float] = None, max_computing: Optional[int] = None):
         self.pool = pool
         self.max_inflight = max_inflight
+        self.cycle_share = (1.0 / max_inflight) if cycle_share is None else cycle_share
+        self.max_computing = max_inflight if max_computing is None else max_computing
         self.clock = clock
         self.on_done = on_done      # (key, session id, arrival time) once a tick's results are in HBM
         self.waiting: Deque[Tuple[Hashable, int]] = deque()
@@ -91,13 +104,19 @@ class TickServer:
         """Submit one cycle of every ready session, if a cycle slot is free."""
         if not self.ready or len(self.inflight) >= self.max_inflight:
             return False
+        if self.max_computing < self.max_inflight and \
+                sum(1 for c, _ in self.inflight if not self.pool.computed(c)) >= self.max_computing:
+            return False
         ticks, members = [], []
-        for key in self.ready:
+        cap = max(1, math.ceil(len(self.sid) * self.cycle_share))
+        for key in list(self.ready):
+            if len(members) >= cap:
+                break
             t0, rows = self.queue[key].popleft()
             ticks.append(rows if isinstance(rows, kr.StreamPool.Rows) else (self.sid[key], rows))
             members.append((key, t0))
             self.busy.add(key)
-        self.ready.clear()
+            del self.ready[key]
         t = self.clock()
         cyc = self.pool.submit(ticks)
         self.submit_s += self.clock() - t
@@ -108,10 +127,16 @@ class TickServer:
 
     def step(self):
         self.poll()
-        if not self.dispatch() and self.inflight:
+        if self.dispatch() or not self.inflight:
+            return
+        computing = [c for c, _ in self.inflight if not self.pool.computed(c)]
+        if self.ready and len(computing) >= self.max_computing and len(self.inflight) < self.max_inflight:
+            # ticks wait for the computing cycle's phases only
+            self.pool.computed(computing[-1], wait=True)
+        else:
             # nothing to start: wait for the oldest cycle
             self.pool.done(self.inflight[0][0], wait=True)
-            self.poll()
+        self.poll()
 
     def drain(self):
         while self.inflight or self.ready:

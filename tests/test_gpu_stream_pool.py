@@ -134,3 +134,86 @@ def test_pool_admission_by_footprint(executor):
             assert _same(pool.read(b, k), alone_b[t][k]), (t, k)
     with pytest.raises(RuntimeError, match="twice"):
         pool.submit([(b, rows_b[0]), (b, rows_b[0])])
+
+
+def test_pool_readback_and_phase_times(executor):
+    """With readback buffers set, a finished cycle's results are in pinned
+    host memory: they equal a stream context's outputs tick by tick (two
+    cycles in flight, sessions sitting out at random). Profiled cycles give
+    one time per phase plus input staging and readback."""
+    from kunquant_amd.codegen.hip import stream_phases
+    _, mod = catalog.load("serve_f32")
+    outs = mod.getOutputNames()
+    sizes, T = (72, 128, 40), 10
+    rng = np.random.default_rng(83)
+    feeds = [_ticks(mod, S, T, seed=70 + j) for j, S in enumerate(sizes)]
+    alone = [_alone(executor, mod, S, f, outs) for S, f in zip(sizes, feeds)]
+    pool = kr.StreamPool(executor, mod, capacity_stocks=512, max_sessions=4, max_inflight=2)
+    sids = [pool.admit(S) for S in sizes]
+    pool.set_readback(outs)
+    pool.profile(True)
+    done, pending, times = [0] * 3, [], []
+    while min(done) < T:
+        members = [j for j in range(3) if done[j] < T and not any(j in m for _, m in pending)
+                   and rng.random() < 0.8]
+        if members:
+            pending.append((pool.submit([(sids[j], feeds[j][done[j]]) for j in members]), members))
+        if len(pending) == 2 or (not members and pending):
+            cyc, m = pending.pop(0)
+            assert pool.done(cyc, wait=True)
+            for j in m:
+                bad = [k for k in outs if not _same(pool.result(cyc, sids[j], k), alone[j][done[j]][k])]
+                assert not bad, f"session {j} tick {done[j]}: {bad[:3]}"
+                done[j] += 1
+            times.append(pool.phase_times(cyc))
+    for cyc, _ in pending:
+        pool.done(cyc, wait=True)
+    nph = len(stream_phases(catalog.module_ir("serve_f32")))
+    assert all(len(t) == nph + 2 for t in times)
+    assert all((t >= 0).all() and t[-1] > 0 for t in times)
+    with pytest.raises(RuntimeError, match="not a readback buffer"):
+        pool.result(cyc, sids[0], mod.getInputNames()[0])
+    with pytest.raises(RuntimeError, match="replaced"):
+        pool.result(1, sids[0], outs[0])
+    pool.set_readback([])
+    c = pool.submit([(sids[0], feeds[0][0])])
+    pool.done(c, wait=True)
+    assert pool.phase_times(c)[-1] < 0.05  # no readback: nothing between the last two marks
+
+
+def test_tick_server_with_cycle_cap(executor):
+    """The request loop ticks each session in arrival order, cycles holding
+    at most half the sessions (two in flight): every tick's results equal a
+    stream context's, whatever cycle it ran in."""
+    from kunquant_amd.serving import TickServer
+    _, mod = catalog.load("stream_f32")
+    outs = mod.getOutputNames()
+    sizes, T = (72, 8, 136, 64, 40), 12
+    feeds = [_ticks(mod, S, T, seed=60 + j) for j, S in enumerate(sizes)]
+    alone = [_alone(executor, mod, S, f, outs) for S, f in zip(sizes, feeds)]
+    pool = kr.StreamPool(executor, mod, capacity_stocks=1024, max_sessions=8, max_inflight=2)
+    got = {j: [] for j in range(len(sizes))}
+    server = TickServer(pool, max_inflight=2,
+                        on_done=lambda key, sid, t0: got[key].append({k: pool.read(sid, k) for k in outs}))
+    for j, S in enumerate(sizes):
+        assert server.open(j, S)
+    rng = np.random.default_rng(61)
+    order = [j for j in range(len(sizes)) for _ in range(T)]
+    rng.shuffle(order)
+    nxt = [0] * len(sizes)
+    sizes_in_cycles = []
+    for j in order:
+        server.push(j, feeds[j][nxt[j]])
+        nxt[j] += 1
+        before = server.ticks_in_cycles
+        server.poll()
+        server.dispatch()
+        if server.ticks_in_cycles > before:
+            sizes_in_cycles.append(server.ticks_in_cycles - before)
+    server.drain()
+    assert max(sizes_in_cycles) <= 3  # ceil(5 sessions / 2 in flight)
+    for j in range(len(sizes)):
+        assert len(got[j]) == T
+        for t in range(T):
+            bad = [k for k in outs if not _same(got[j][t][k], alone[j][t][k])]
+            assert not bad, f"session {j} tick {t}: {bad[:3]}"

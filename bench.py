@@ -58,6 +58,8 @@ def main():
     ap.add_argument("--no-host-baseline", action="store_true",
                     help="--workload jobs: skip the sequential host-buffer runGraph pass")
     ap.add_argument("--no-graph", action="store_true", help="--workload jobs: skip the batch as one HIP graph")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="default workload only: skip the other workloads' lines embedded under other_workloads")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 1 GiB device-to-device three times (calibrates rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -74,6 +76,11 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the factor pipeline "
               f"shards by time or by job across GPUs, which this build does not run.", file=sys.stderr)
         sys.exit(2)
+    extra = None
+    if args.workload == "factors" and args.module == "cfg2_f32" and not args.no_extra:
+        # the other workloads, each in a child process started before this
+        # one touches the GPU; their lines are embedded in this one
+        extra = other_workloads()
     if args.workload == "jobs":
         run_jobs(args)
         return
@@ -301,7 +308,46 @@ def main():
             base["sample"] = f"same graph and data, first {rows} of {T} days, {S} stocks " + base["sample"]
         base["host"] = host_cpu()
         result["cpu_baseline"] = base
+    if extra is not None:
+        result["other_workloads"] = extra
     print(json.dumps(result), flush=True)
+
+
+# the other workloads of the benchmark, as run by `python bench.py` with no
+# flags: (name, arguments, time limit in seconds)
+OTHER_WORKLOADS = [
+    ("fp64_100_factors", ["--module", "cfg2_f64", "--no-e2e"], 300),
+    ("alpha158_fp64", ["--module", "alpha158_f64", "--no-e2e"], 300),
+    ("jobs64_uneven", ["--workload", "jobs"], 400),
+    ("serve_256_sessions", ["--workload", "serve"], 400),
+    ("precision_150_factors", ["--workload", "precision"], 400),
+]
+
+
+def other_workloads():
+    """Each other workload's JSON line (bulky per-factor tables dropped),
+    from a child process; a failure is recorded, not raised."""
+    import subprocess
+    out = {}
+    for name, extra_args, limit in OTHER_WORKLOADS:
+        cmd = [sys.executable, os.path.abspath(__file__), "--no-extra"] + extra_args
+        t0 = time.perf_counter()
+        print(f"bench: {name} ...", file=sys.stderr, flush=True)
+        try:
+            r = subprocess.run(cmd, capture_output=True, text=True, timeout=limit)
+            line = [x for x in r.stdout.strip().splitlines() if x.startswith("{")]
+            if r.returncode != 0 or not line:
+                out[name] = {"error": f"exit {r.returncode}: {r.stderr.strip()[-400:]}"}
+                continue
+            rec = json.loads(line[-1])
+            rec.get("precision", {}).pop("per_factor_max_err", None)
+            rec["wall_s"] = time.perf_counter() - t0
+            rec["command"] = "python bench.py " + " ".join(extra_args)
+            out[name] = rec
+        except subprocess.TimeoutExpired:
+            out[name] = {"error": f"time limit {limit} s"}
+        print(f"bench: {name} done in {time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
+    return out
 
 
 def dag_timeline(kr, mod, ptrs, S, T, wall_ms):

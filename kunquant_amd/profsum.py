@@ -4,6 +4,7 @@
     python -m kunquant_amd.profsum valu   COUNTER_CSV       [--title T] [--command C] [--json OUT]
     python -m kunquant_amd.profsum traffic FETCH_CSV WRITE_CSV COPY_FETCH_CSV COPY_WRITE_CSV
                                     --module M --stocks S --days T --json OUT [--command C]
+    python -m kunquant_amd.profsum traffic-md TRAFFIC_JSON [--top N]
 
 `trace` reads `rocprofv3 --kernel-trace --output-format csv` and prints a
 markdown table of kernels (calls, total and average duration) and, per batch
@@ -102,7 +103,9 @@ def trace(args):
             "in flight = summed kernel time / busy time.", "",
             "| run | kernels | wall ms | GPU-busy | idle ms | in flight | summed ms by class |",
             "|---|---|---|---|---|---|---|"]
-    for i, run in enumerate(runs):
+    if len(runs) > args.max_runs:
+        out.append(f"(first {args.max_runs} of {len(runs)} runs)")
+    for i, run in enumerate(runs[:args.max_runs]):
         t0, t1 = run[0][0], max(e for _, e, _ in run)
         busy, hi = 0, t0
         for s, e, _ in run:
@@ -165,8 +168,11 @@ def valu(args):
                    f"{r['valu_rate'] / 1e9:.1f} | {r['valu_frac']:.3f} | {act} | {lds} |")
     print("\n".join(out))
     if args.json:
+        keep = ("dispatches", "avg_us", "valu_insts", "valu_frac")
         rec = {"command": args.command, "peak_valu_insts_per_s": VALU_PEAK, "commit": args.commit or _commit(),
-               "runtime_sha256": _sha(_runtime_lib()), "kernels": {r["kernel"]: r for r in rows}}
+               "runtime_sha256": _sha(_runtime_lib()),
+               "kernels": {r["kernel"]: {k: (round(r[k], 4) if isinstance(r[k], float) else r[k]) for k in keep}
+                           for r in rows}}
         if args.module:
             rec["module_sha256"] = _sha(_module_lib(args.module))
         with open(args.json, "w") as f:
@@ -223,7 +229,7 @@ def traffic(args):
     sf = gib_kib / _copy_kib(args.copy_fetch, "FETCH_SIZE")
     sw = gib_kib / _copy_kib(args.copy_write, "WRITE_SIZE")
     f, w = _per_dispatch(args.fetch, "FETCH_SIZE"), _per_dispatch(args.write, "WRITE_SIZE")
-    kern = {k: {"fetch_bytes": f[k] * 1024 * sf, "write_bytes": w.get(k, 0.0) * 1024 * sw} for k in f}
+    kern = {k: {"fetch_bytes": round(f[k] * 1024 * sf), "write_bytes": round(w.get(k, 0.0) * 1024 * sw)} for k in f}
     sums = [k for k in kern if k.startswith("scale_sums<")]
     divs = [k for k in kern if k.startswith("scale_div<")]
     if sums and divs:  # one Scale stage = row sums + division, as bench.py groups them
@@ -238,6 +244,24 @@ def traffic(args):
     print(f"{args.json}: {len(kern)} kernels; copy scale fetch {sf:.3f}, write {sw:.3f}")
 
 
+def traffic_md(args):
+    with open(args.json) as f:
+        rec = json.load(f)
+    k = rec["kernels"]
+    rows = sorted(k.items(), key=lambda kv: -(kv[1]["fetch_bytes"] + kv[1]["write_bytes"]))
+    out = [f"# HBM traffic per launch, module {rec['module']}, {rec['stocks']} stocks x {rec['days']} days", "",
+           "Command (from the repository root):", "", "    " + (rec.get("command") or ""), "",
+           f"Recorded at commit {rec.get('commit')}; runtime library sha256 {rec['runtime_sha256'][:16]}..., "
+           f"module library sha256 {rec['module_sha256'][:16]}... (`bench.py` quotes a kernel's figure only "
+           f"when the library it runs has that hash).", "",
+           f"Calibration: FETCH_SIZE x {rec['scale_fetch']:.3f}, WRITE_SIZE x {rec['scale_write']:.3f} "
+           f"(a 1 GiB device copy read back through the same counters).", "",
+           "| kernel | MB read | MB written |", "|---|---|---|"]
+    for n, v in rows[:args.top]:
+        out.append(f"| {n} | {v['fetch_bytes'] / 1e6:.1f} | {v['write_bytes'] / 1e6:.1f} |")
+    print("\n".join(out))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m kunquant_amd.profsum")
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -245,6 +269,7 @@ def main(argv=None):
     t.add_argument("csv")
     t.add_argument("--launches", type=int, default=0, help="kernel launches per run (runs back to back)")
     t.add_argument("--runs", type=int, default=0, help="number of runs the library's launches divide into")
+    t.add_argument("--max-runs", type=int, default=12, help="rows of the timeline table")
     v = sub.add_parser("valu")
     v.add_argument("csv")
     v.add_argument("--json")
@@ -263,8 +288,11 @@ def main(argv=None):
     r.add_argument("--json", required=True)
     r.add_argument("--command", default="")
     r.add_argument("--commit", help="commit the libraries were built from (default: git HEAD)")
+    tm = sub.add_parser("traffic-md")
+    tm.add_argument("json")
+    tm.add_argument("--top", type=int, default=20)
     a = ap.parse_args(argv)
-    {"trace": trace, "valu": valu, "traffic": traffic}[a.cmd](a)
+    {"trace": trace, "valu": valu, "traffic": traffic, "traffic-md": traffic_md}[a.cmd](a)
 
 
 if __name__ == "__main__":

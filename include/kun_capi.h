@@ -148,11 +148,13 @@ KUN_API int kunRunGraphBatchDevice(KunExecutorHandle exec, int njobs, const KunM
                                    const KunBufferNameMapHandle *bufs, const size_t *num_stocks,
                                    const size_t *total_time, const size_t *cur_time, const size_t *length,
                                    void *hip_stream);
-/* The same batch as one HIP graph (with temporaries of its own), replayed
- * by kunLaunchGraphBatch with one hipGraphLaunch for every kernel of every
- * job. The graph is built node by node from the jobs' launch DAGs (no
- * stream capture, no events inside it). The buffers' device pointers are
- * baked in. NULL on failure (kunGetLastError). */
+/* The same batch as HIP graphs (with temporaries of their own), replayed by
+ * kunLaunchGraphBatch: the jobs split into groups of about equal work, as
+ * many as the executor has lanes, one graph per group, each launched once
+ * per replay on a stream of the batch's own (forked from and joined into
+ * the replay stream). The graphs are built node by node from the jobs'
+ * launch DAGs (no stream capture, no events inside them). The buffers'
+ * device pointers are baked in. NULL on failure (kunGetLastError). */
 KUN_API KunBatchGraphHandle kunCaptureGraphBatch(KunExecutorHandle exec, int njobs, const KunModuleHandle *mods,
                                                  const KunBufferNameMapHandle *bufs, const size_t *num_stocks,
                                                  const size_t *total_time, const size_t *cur_time,

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "kun_module_abi.h"
+#include "lane_sync.hpp"
 
 #ifndef KUN_MAX_COMPS
 #define KUN_MAX_COMPS 64
@@ -89,6 +90,10 @@ struct BatchLayout {
         double est = 0; // estimated microseconds
     };
     std::vector<Node> nodes;
+    // cross-lane synchronisation of one run of these nodes on `sync_lanes`
+    // lanes (plan_lane_sync), computed at the first run
+    LaneSync sync;
+    int sync_lanes = -1;
 };
 // How PAR kernels split their rows (defaults overridable by the environment
 // variables KUN_TARGET_WORKGROUPS, KUN_HALO_FACTOR, KUN_MIN_CHUNK_ROWS).
@@ -187,12 +192,18 @@ struct BatchJob {
 };
 void run_jobs_device(Executor &ex, const std::vector<BatchJob> &jobs, hipStream_t st);
 
-// a batch of jobs as one HIP graph, with temporaries of its own
+// a batch of jobs as HIP graphs with temporaries of their own: the jobs
+// split into groups of about equal work, one graph per group, replayed side
+// by side on the batch's own streams (forked from and joined into the
+// caller's stream)
 struct CapturedBatch {
     int device = 0;
     std::shared_ptr<Executor> ex; // its stream is the default replay stream
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    std::vector<hipGraph_t> graphs;
+    std::vector<hipGraphExec_t> execs;
+    std::vector<hipStream_t> streams; // one per graph
+    std::vector<hipEvent_t> joined;   // one per graph, recorded after its replay
+    hipEvent_t fork = nullptr;
     std::vector<DeviceTemps> temps;
     // recorded after each replay; a replay on another stream waits for it
     hipEvent_t done = nullptr;

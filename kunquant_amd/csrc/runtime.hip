@@ -599,16 +599,26 @@ void enqueue_batch(Executor &ex, const KunModuleDesc *m, const std::vector<void 
         enqueue_batch_serial(m, dev, lay, arena, S, T, start, L, st, prof);
         return;
     }
+    if (lay.sync_lanes != nl) {
+        std::vector<int> lane(lay.nodes.size());
+        std::vector<std::vector<int>> deps(lay.nodes.size());
+        for (size_t n = 0; n < lay.nodes.size(); n++) {
+            lane[n] = lay.nodes[n].lane % nl;
+            deps[n] = lay.nodes[n].deps;
+        }
+        BatchLayout &w = const_cast<BatchLayout &>(lay); // a cache of the layout's own plan
+        w.sync = plan_lane_sync(lane, deps, nl);
+        w.sync_lanes = nl;
+    }
     ex.ensure_stage_events((int)lay.nodes.size());
     HIPCHECK(hipEventRecord(ex.fork, st));
     for (int l = 0; l < nl; l++) HIPCHECK(hipStreamWaitEvent(ex.lanes[l], ex.fork, 0));
     for (size_t n = 0; n < lay.nodes.size(); n++) {
         const auto &nd = lay.nodes[n];
         hipStream_t ls = ex.lanes[nd.lane % nl];
-        for (int d : nd.deps)
-            if (lay.nodes[d].lane % nl != nd.lane % nl) HIPCHECK(hipStreamWaitEvent(ls, ex.stage_done[d], 0));
+        for (int d : lay.sync.waits[n]) HIPCHECK(hipStreamWaitEvent(ls, ex.stage_done[d], 0));
         enqueue_node(m, nd.stage, nd.kern, dev, lay, arena, S, T, start, L, ls, prof);
-        HIPCHECK(hipEventRecord(ex.stage_done[n], ls));
+        if (lay.sync.record[n]) HIPCHECK(hipEventRecord(ex.stage_done[n], ls));
     }
     for (int l = 0; l < nl; l++) {
         HIPCHECK(hipEventRecord(ex.lane_done[l], ex.lanes[l]));
@@ -795,25 +805,37 @@ static void enqueue_jobs(Executor &ex, const std::vector<BatchJob> &jobs, const 
     ex.ensure_stage_events((int)first.back());
     HIPCHECK(hipEventRecord(ex.fork, st));
     for (int l = 0; l < nl; l++) HIPCHECK(hipStreamWaitEvent(ex.lanes[l], ex.fork, 0));
+    // enqueue order: one node of each job in turn (job j shifted j lanes),
+    // items numbered in that order; events are indexed by job and node
+    std::vector<std::pair<size_t, size_t>> order; // (job, node)
     std::vector<size_t> next(jobs.size(), 0);
-    bool more = true;
-    while (more) {
+    for (bool more = true; more;) {
         more = false;
-        for (size_t j = 0; j < jobs.size(); j++) {
-            const auto &nodes = temps[j]->lay.nodes;
-            if (next[j] >= nodes.size()) continue;
-            more = true;
-            const size_t n = next[j]++;
-            const auto &nd = nodes[n];
-            const int lane = (int)((nd.lane + j) % nl);
-            hipStream_t ls = ex.lanes[lane];
-            for (int d : nd.deps)
-                if ((int)((nodes[d].lane + j) % nl) != lane)
-                    HIPCHECK(hipStreamWaitEvent(ls, ex.stage_done[first[j] + d], 0));
-            enqueue_node(jobs[j].m, nd.stage, nd.kern, dev[j], temps[j]->lay, temps[j]->arena, jobs[j].S, jobs[j].T,
-                         jobs[j].start, jobs[j].L, ls, prof);
-            HIPCHECK(hipEventRecord(ex.stage_done[first[j] + n], ls));
-        }
+        for (size_t j = 0; j < jobs.size(); j++)
+            if (next[j] < temps[j]->lay.nodes.size()) {
+                order.emplace_back(j, next[j]++);
+                more = true;
+            }
+    }
+    std::vector<int> item_of(first.back()), lane(order.size());
+    std::vector<std::vector<int>> deps(order.size());
+    for (size_t i = 0; i < order.size(); i++) {
+        const size_t j = order[i].first, n = order[i].second;
+        const auto &nd = temps[j]->lay.nodes[n];
+        item_of[first[j] + n] = (int)i;
+        lane[i] = (int)((nd.lane + j) % nl);
+        for (int d : nd.deps) deps[i].push_back(item_of[first[j] + d]);
+    }
+    const LaneSync sync = plan_lane_sync(lane, deps, nl);
+    for (size_t i = 0; i < order.size(); i++) {
+        const size_t j = order[i].first, n = order[i].second;
+        const auto &nd = temps[j]->lay.nodes[n];
+        hipStream_t ls = ex.lanes[lane[i]];
+        for (int d : sync.waits[i])
+            HIPCHECK(hipStreamWaitEvent(ls, ex.stage_done[first[order[d].first] + order[d].second], 0));
+        enqueue_node(jobs[j].m, nd.stage, nd.kern, dev[j], temps[j]->lay, temps[j]->arena, jobs[j].S, jobs[j].T,
+                     jobs[j].start, jobs[j].L, ls, prof);
+        if (sync.record[i]) HIPCHECK(hipEventRecord(ex.stage_done[first[j] + n], ls));
     }
     for (int l = 0; l < nl; l++) {
         HIPCHECK(hipEventRecord(ex.lane_done[l], ex.lanes[l]));
@@ -840,9 +862,9 @@ void run_jobs_device(Executor &ex, const std::vector<BatchJob> &jobs, hipStream_
 }
 
 // ---------------------------------------------------------------------------
-// A batch of jobs as one HIP graph, built node by node (hipGraphAddKernelNode)
-// from the jobs' launch DAGs rather than captured from streams: no events,
-// no stream capture. Each DAG node contributes its kernels, the first
+// A batch of jobs as HIP graphs, built node by node (hipGraphAddKernelNode)
+// from the jobs' launch DAGs rather than captured from streams: no events
+// inside the graphs, no stream capture. Each DAG node contributes its kernels, the first
 // depending on the last kernel of each of the node's dependencies and of the
 // node before it on the same lane, later ones on the kernel before them;
 // jobs are independent subgraphs. The lane edges matter: the arena's memory
@@ -935,16 +957,47 @@ CapturedBatch *capture_jobs(std::shared_ptr<Executor> exp, const std::vector<Bat
     cb->device = ex.device;
     cb->ex = exp;
     cb->temps.reserve(jobs.size());
-    std::vector<const DeviceTemps *> temps;
-    for (const auto &jb : jobs) {
-        cb->temps.push_back(make_temps(ex, jb.m, jb.S, jb.T, jb.L));
-        temps.push_back(&cb->temps.back());
+    for (const auto &jb : jobs) cb->temps.push_back(make_temps(ex, jb.m, jb.S, jb.T, jb.L));
+    // groups of about equal work (stocks x rows x stages, largest first onto
+    // the lightest group), as many as the executor has lanes
+    const size_t K = std::max<size_t>(1, std::min(ex.lanes.size(), jobs.size()));
+    std::vector<size_t> order(jobs.size());
+    for (size_t j = 0; j < jobs.size(); j++) order[j] = j;
+    auto work = [&](size_t j) { return (double)jobs[j].S * (double)jobs[j].L * (double)jobs[j].m->nstage; };
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return work(a) > work(b); });
+    std::vector<std::vector<size_t>> groups(K);
+    std::vector<double> load(K, 0.0);
+    for (size_t j : order) {
+        if (jobs[j].L == 0) continue;
+        const size_t k = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
+        groups[k].push_back(j);
+        load[k] += work(j);
     }
     HIPCHECK(hipEventCreateWithFlags(&cb->done, hipEventDisableTiming));
-    HIPCHECK(hipGraphCreate(&cb->graph, 0));
-    HipGraphBuilder gb(cb->graph);
-    build_jobs_graph(gb, jobs, temps);
-    HIPCHECK(hipGraphInstantiate(&cb->exec, cb->graph, nullptr, nullptr, 0));
+    HIPCHECK(hipEventCreateWithFlags(&cb->fork, hipEventDisableTiming));
+    for (const auto &grp : groups) {
+        if (grp.empty()) continue;
+        std::vector<BatchJob> gj;
+        std::vector<const DeviceTemps *> gt;
+        for (size_t j : grp) {
+            gj.push_back(jobs[j]);
+            gt.push_back(&cb->temps[j]);
+        }
+        hipGraph_t graph = nullptr;
+        HIPCHECK(hipGraphCreate(&graph, 0));
+        cb->graphs.push_back(graph);
+        HipGraphBuilder gb(graph);
+        build_jobs_graph(gb, gj, gt);
+        hipGraphExec_t exec = nullptr;
+        HIPCHECK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        cb->execs.push_back(exec);
+        hipStream_t s;
+        HIPCHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        cb->streams.push_back(s);
+        hipEvent_t e;
+        HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        cb->joined.push_back(e);
+    }
     return cb.release();
 }
 
@@ -953,15 +1006,25 @@ void launch_captured(CapturedBatch &cb, hipStream_t st) {
     if (!st) st = cb.ex->stream;
     // replays on other streams wait for the previous replay (shared temporaries)
     if (cb.armed) HIPCHECK(hipStreamWaitEvent(st, cb.done, 0));
-    HIPCHECK(hipGraphLaunch(cb.exec, st));
+    HIPCHECK(hipEventRecord(cb.fork, st));
+    for (size_t k = 0; k < cb.execs.size(); k++) {
+        HIPCHECK(hipStreamWaitEvent(cb.streams[k], cb.fork, 0));
+        HIPCHECK(hipGraphLaunch(cb.execs[k], cb.streams[k]));
+        HIPCHECK(hipEventRecord(cb.joined[k], cb.streams[k]));
+    }
+    for (auto e : cb.joined) HIPCHECK(hipStreamWaitEvent(st, e, 0));
     HIPCHECK(hipEventRecord(cb.done, st));
     cb.armed = true;
 }
 
 CapturedBatch::~CapturedBatch() {
     if (done) hipEventSynchronize(done);
-    if (exec) hipGraphExecDestroy(exec);
-    if (graph) hipGraphDestroy(graph);
+    for (auto s : streams) hipStreamSynchronize(s);
+    for (auto e : execs) hipGraphExecDestroy(e);
+    for (auto g : graphs) hipGraphDestroy(g);
+    for (auto s : streams) hipStreamDestroy(s);
+    for (auto e : joined) hipEventDestroy(e);
+    if (fork) hipEventDestroy(fork);
     if (done) hipEventDestroy(done);
     for (auto &t : temps) free_device_temps(t);
 }

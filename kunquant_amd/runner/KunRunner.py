@@ -190,9 +190,10 @@ def createMultiThreadExecutor(num_threads: int) -> Executor:
 
 
 class Module:
-    def __init__(self, handle, lib: "Library"):
+    def __init__(self, handle, lib: "Library", name: str = None):
         self._h = handle
         self._lib = lib  # keeps the library loaded
+        self.name = name
         L = _load()
         self.dtype = np.float64 if L.kunModuleDtype(handle) == 1 else np.float32
         self.input_layout = _LAYOUTS[L.kunModuleInputLayout(handle)]
@@ -281,7 +282,7 @@ class Library:
         h = _load().kunGetModuleFromLibrary(self._h, name.encode())
         if not h:
             raise RuntimeError("Module name not found")
-        return Module(h, self)
+        return Module(h, self, name)
 
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:

@@ -112,3 +112,26 @@ def test_device_runs_on_two_streams_share_temporaries_safely(executor):
         exp = prog.run({k: d[k] for k in mod.getInputNames()}, S, T, 0, T, nthreads=16)
         bad = [k for k in exp if not np.array_equal(dev_out[k].cpu().numpy(), exp[k], equal_nan=True)]
         assert not bad, f"run {i} on its own stream: outputs {bad[:3]} differ from the oracle"
+
+
+def test_autotune_cache(executor, tmp_path):
+    """A pick stored in the cache file is applied by a later tune() without
+    timing anything; results do not depend on the policy."""
+    import torch
+    from kunquant_amd import autotune
+    _, mod = catalog.load("simple_f32")
+    S, T = 256, 200
+    d = data.ohlcv(S, T, seed=9)
+    dev = {k: torch.from_numpy(d[k]).cuda() for k in mod.getInputNames()}
+    out = {k: torch.empty((T, S), dtype=torch.float32, device="cuda") for k in mod.getOutputNames()}
+    ptrs = {k: v.data_ptr() for k, v in list(dev.items()) + list(out.items())}
+    cache = str(tmp_path / "tune.json")
+    best, times = autotune.tune(executor, mod, ptrs, S, T, reps=1, cache=cache)
+    assert times and best in times
+    first = {k: v.cpu().numpy() for k, v in out.items()}
+    again, t2 = autotune.tune(executor, mod, ptrs, S, T, reps=1, cache=cache)
+    assert again == best and t2 == {}
+    kr.runGraphDevice(executor, mod, ptrs, S, T, 0, T)
+    executor.synchronize()
+    for k, v in out.items():
+        assert np.array_equal(v.cpu().numpy(), first[k], equal_nan=True), k

@@ -21,6 +21,7 @@ Stream stage kernel (STREAM modules): the same op lowering for one row;
 rings and carried state live in HBM as [slot][S] arrays owned by the stream
 context (the reference's StreamBuffer / StateBuffer, Runtime.cpp:384-546).
 Cross-sectional stages (Rank / Scale) are runtime library kernels."""
+import functools
 import math
 import os
 import re
@@ -90,6 +91,32 @@ def _pow2(n: int) -> int:
     while r < n:
         r <<= 1
     return r
+
+
+@functools.lru_cache(maxsize=None)
+def div_product_exact(c: float) -> bool:
+    """True when (float)((double)x * RN(1/c)) is the IEEE float quotient x / c
+    for every float x (c a finite nonzero float). The product is within
+    2^-52 of x / c, which is never that near a rounding midpoint unless it
+    is one exactly (see kun::div_rc); exact midpoints exist only between
+    subnormal floats, for c = C * 2^k with C odd and k >= 1: x = o * C *
+    2^(k-150) for odd o with o * C < 2^24. Each such x is tried."""
+    n, d = float(c).as_integer_ratio()
+    n = abs(n)
+    if d != 1 or n & 1:
+        return True            # k < 1: no float x is a midpoint times c
+    k = (n & -n).bit_length() - 1
+    C = n >> k
+    if C == 1:
+        return True            # power of two: the product is exact
+    rc = 1.0 / float(c)
+    o = np.arange(1, (1 << 24) // C + 1, 2, dtype=np.int64)
+    o = o[o * C < (1 << 24)]
+    x = (o * C).astype(np.float64) * 2.0 ** (k - 150)     # exact, and floats
+    with np.errstate(all="ignore"):
+        want = x.astype(np.float32) / np.float32(c)
+        got = (x * rc).astype(np.float32)
+    return bool(np.array_equal(want.view(np.uint32), got.view(np.uint32)))
 
 
 def ident(name: str) -> str:
@@ -164,13 +191,23 @@ def loop_code(w: int, body: List[str], unroll: int) -> List[str]:
     return L
 
 
+def window_loop(w: int, body: List[str], fast: Optional[List[str]], unroll: int, row: str = "i") -> List[str]:
+    """loop_code of a window loop; with an in-run body, that copy runs when
+    every row the loop reads lies in the run (`row` >= w - 1, `row` being
+    the lowest row of the loop) and the buffers allow kun::wld."""
+    if fast is None:
+        return loop_code(w, body, unroll)
+    return ([f"if (wok && {row} >= {w - 1}) {{"] + loop_code(w, fast, unroll) + ["} else {"]
+            + loop_code(w, body, unroll) + ["}"])
+
+
 def _flatten(segs, unroll: int) -> List[str]:
     L = []
     for sg in segs:
         if sg[0] == "lines":
             L.extend(sg[1])
         else:
-            L.extend(sg[1] + loop_code(sg[2], sg[3], unroll) + sg[4])
+            L.extend(sg[1] + window_loop(sg[2], sg[3], sg[5], unroll) + sg[4])
     return L
 
 
@@ -271,8 +308,11 @@ class _Lowering:
             cf = float(np.float32(op.attrs["value"]))
             if name == "DivConst" and T == "float" and not op.attrs.get("swap", False) \
                     and math.isfinite(cf) and cf != 0.0:
-                # float division by a constant as a double product (exact, see kun::div_const)
-                expr = f"kun::div_const({ins[0]}, {float.hex(1.0 / cf)})"
+                # float division by a constant as a double product, exact:
+                # bare where no midpoint quotient exists or all round right
+                # (div_product_exact), else subnormal results divide
+                fn = "kun::div_const" if div_product_exact(cf) else "kun::div_rc"
+                expr = f"{fn}({ins[0]}, {float.hex(1.0 / cf)}, {c})"
             if name.endswith("ThanConst"):
                 expr = f"kun::mask<{T}>{expr}"
             L.append(f"const {T} {out} = {expr};")
@@ -349,13 +389,15 @@ class _Lowering:
         L.extend(self.after_value(k))
         return L
 
-    def loop_lines(self, loop) -> List[str]:
-        pre, w, body, post = self.loop_parts(loop)
-        return pre + loop_code(w, body, self.loop_unroll) + post
+    # None: the emitter has no in-run copy of window loops
+    fast_win = None
 
     def loop_parts(self, loop):
-        """(declarations, window, loop body, finalisation) of a window loop
-        (see loop_code)."""
+        """(declarations, window, loop body, finalisation, in-run body) of a
+        window loop (see loop_code). The in-run body, when the emitter has
+        one and it differs, reads row sources without the clamp and NaN
+        select that rows before the run start need; it is correct for rows
+        i >= window - 1 and reads the same values there."""
         T = self.T
         w = loop.attrs["window"]
         body = [op for op in self.ops if op.get_parent() is loop]
@@ -402,6 +444,34 @@ class _Lowering:
             else:
                 raise RuntimeError(f"HIP emitter: unsupported reduction {name}")
         pre = L
+        body_lines = self._loop_body(body, reds)
+        fast = None
+        if self.fast_win is False:
+            # the copy for rows whose window lies inside the run (row >= w - 1)
+            self.fast_win = True
+            try:
+                fast = self._loop_body(body, reds)
+            finally:
+                self.fast_win = False
+            if fast == body_lines:
+                fast = None
+        L = []
+        for r in reds:
+            k = self.idx[r]
+            name = r.__class__.__name__
+            if name == "ReduceAdd" and self.fin:
+                L.append(f"v{k} = ({T})r{k}_v;")
+            elif name in ("ReduceAdd", "ReduceMul", "ReduceMin", "ReduceMax", "ReduceDecayLinear"):
+                L.append(f"v{k} = r{k}_v;")
+            elif name in ("ReduceArgMax", "ReduceArgMin"):
+                L.append(f"v{k} = r{k}_i;")
+            elif name == "ReduceRank":
+                L.append(f"v{k} = r{k}_lt + (r{k}_eq + {T}(1.0)) / {T}(2.0);")
+            L.extend(self.after_value(k))
+        return pre, w, body_lines, L, fast
+
+    def _loop_body(self, body, reds) -> List[str]:
+        T = self.T
         L = []
         for op in body:
             L.extend("  " + x for x in self.op_lines(op))
@@ -431,21 +501,7 @@ class _Lowering:
                          f" r{k}_eq = ({x} == r{k}_cur) ? r{k}_eq + {T}(1.0) : r{k}_eq; }}")
             elif name == "ReduceDecayLinear":  # Ops.hpp:507-520
                 L.append(f"  r{k}_v = kun::fma({x}, r{k}_w, r{k}_v); r{k}_w = r{k}_w + r{k}_step;")
-        body_lines = L
-        L = []
-        for r in reds:
-            k = self.idx[r]
-            name = r.__class__.__name__
-            if name == "ReduceAdd" and self.fin:
-                L.append(f"v{k} = ({T})r{k}_v;")
-            elif name in ("ReduceAdd", "ReduceMul", "ReduceMin", "ReduceMax", "ReduceDecayLinear"):
-                L.append(f"v{k} = r{k}_v;")
-            elif name in ("ReduceArgMax", "ReduceArgMin"):
-                L.append(f"v{k} = r{k}_i;")
-            elif name == "ReduceRank":
-                L.append(f"v{k} = r{k}_lt + (r{k}_eq + {T}(1.0)) / {T}(2.0);")
-            L.extend(self.after_value(k))
-        return pre, w, body_lines, L
+        return L
 
     def step_segments(self, skip=()) -> list:
         """Per top-level op: ("lines", statements) or ("loop", declarations,
@@ -883,6 +939,23 @@ class CompEmitter(_Lowering):
         unconditional (row clamped at 0) so the row loop has no branch."""
         return f"kun::nan_before({self.src_load(op, f'(({row}) < 0 ? 0 : ({row}))')}, {row})"
 
+    # set while lowering the copy of a window loop that runs when every row
+    # the loop reads lies inside the run (see loop_parts)
+    fast_win = False
+
+    def in_run_load(self, op, row: str) -> str:
+        """Row `row` (>= 0) of a row source: no clamp, no NaN select. [rows][S]
+        buffers load through kun::wld (row offset in a scalar register)."""
+        k = self.idx[op]
+        e = self.esz
+        if isinstance(op, Input):
+            b = self.m.buffer(op.attrs["name"])
+            if self.buf_layout(op) != "1":
+                return self.src_load(op, row)
+            r = f"A.start + ({row})" if b.kind == BUF_INPUT else f"({row})"
+            return f"kun::wld(b{b.idx}, so, ({r}) * A.S * {e})"
+        return f"kun::wld(m{k}, so, ({row}) * A.S * {e})"
+
     def win(self, src, off) -> str:
         k = self.idx[src]
         if src in self.computed:
@@ -893,6 +966,8 @@ class CompEmitter(_Lowering):
             return f"lg{k}_{off}"
         if k in self.src_ring:
             return self.ring_read(k, off)
+        if self.fast_win:
+            return self.in_run_load(src, f"i - ({off})")
         return self.lag_load(src, f"i - ({off})")
 
     def input_expr(self, op) -> str:
@@ -1006,10 +1081,12 @@ class CompEmitter(_Lowering):
                 continue
             for q in range(P):
                 L.extend(_rename_row(segs[q][j][1], q, keep))
-            body = []
+            body, fast = [], []
             for q in range(P):
                 body.extend(_rename_row(segs[q][j][3], q, keep))
-            L.extend(loop_code(sg[2], body, self.loop_unroll))
+                if sg[5] is not None:
+                    fast.extend(_rename_row(segs[q][j][5], q, keep))
+            L.extend(window_loop(sg[2], body, fast if sg[5] is not None else None, self.loop_unroll, "iq0"))
             for q in range(P):
                 L.extend(_rename_row(segs[q][j][4], q, keep))
         return L
@@ -1026,6 +1103,11 @@ class CompEmitter(_Lowering):
         else:
             L.append("const long c0 = 0, c1 = A.L, rbeg = 0;")
             L.append("(void)c0;")
+        # window loops whose rows all lie in the run take their in-run copy
+        # (kun::wld) when every byte offset of the [rows][S] buffers fits
+        # in 31 bits
+        L.append(f"const unsigned so = (unsigned)s * {self.esz}u; (void)so;")
+        L.append(f"const bool wok = (A.T > A.L ? A.T : A.L) * A.S * {self.esz} < 2147483647L; (void)wok;")
         if self.n_t > self.lds_t or self.n_i:
             L.append("const long col = (long)chunk_id * A.S + s;")
             L.append("const long cst = A.S * (long)A.znch[e];")

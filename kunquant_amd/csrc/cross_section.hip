@@ -16,6 +16,7 @@
 #include <stdint.h>
 
 #include <tuple>
+#include <type_traits>
 #include <utility>
 
 #include "kun_cs.hpp"
@@ -677,7 +678,24 @@ constexpr int SCALE_ROWS = 8;
 constexpr int SCALE_SUM_ROWS = 4;
 constexpr int SCALE_TILE = 1024;
 
-// sums != null: store the row sums there and stop
+// Scratch of the split form: the rows' sums, then (float rows) RN(1/sum) in
+// double at the next multiple of 8 bytes
+template <typename T> __device__ __forceinline__ double *scale_rcs(T *sums, long rows) {
+    return (double *)((char *)sums + ((size_t)rows * sizeof(T) + 7) / 8 * 8);
+}
+
+// x / tot, exactly as IEEE division, or NaN where x == 0 and tot == 0; float
+// rows multiply by rc = RN(1/tot) in double (kun::div_rc)
+__device__ __forceinline__ float scale_q(float x, float tot, double rc) {
+    const float q = kun::div_rc(x, rc, tot);
+    return (x == 0.f && tot == 0.f) ? kun::qnan<float>() : q;
+}
+__device__ __forceinline__ double scale_q(double x, double tot, double) {
+    return (x == 0.0 && tot == 0.0) ? kun::qnan<double>() : x / tot;
+}
+
+// sums != null: store the row sums there (and their reciprocals, float)
+// and stop
 template <typename T, int LIN, int LOUT, int ROWS>
 __device__ __forceinline__ void scale_block(const kun_cs::View &in, const kun_cs::View &out, long S, long rows,
                                             int blk, const long r0, T *sums) {
@@ -727,17 +745,21 @@ __device__ __forceinline__ void scale_block(const kun_cs::View &in, const kun_cs
         }
     }
     if (sums) {
-        if ((int)threadIdx.x < nr) sums[r0 + threadIdx.x] = sum;
+        if ((int)threadIdx.x < nr) {
+            sums[r0 + threadIdx.x] = sum;
+            if (sizeof(T) == 4) scale_rcs(sums, rows)[r0 + threadIdx.x] = 1.0 / (double)sum;
+        }
         return;
     }
     if ((int)threadIdx.x < nr) total[threadIdx.x] = sum;
     __syncthreads();
     for (int r = 0; r < nr; r++) {
         const T tot = total[r];
+        const double rc = sizeof(T) == 4 ? 1.0 / (double)tot : 0.0;
         const long rr = r0 + r;
         for (long s = threadIdx.x; s < S; s += SCALE_BLOCK) {
             const T x = src[lay_at<LIN>(in, rr + in.row0, s, S, blk)];
-            dst[lay_at<LOUT>(out, rr + out.row0, s, S, blk)] = (x == T(0) && tot == T(0)) ? kun::qnan<T>() : x / tot;
+            dst[lay_at<LOUT>(out, rr + out.row0, s, S, blk)] = scale_q(x, tot, rc);
         }
     }
 }
@@ -753,17 +775,37 @@ __global__ __launch_bounds__(SCALE_BLOCK) void scale_sums(kun_cs::View in, long 
     scale_block<T, LIN, LIN, SCALE_SUM_ROWS>(in, in, S, rows, blk, (long)blockIdx.x * SCALE_SUM_ROWS, sums);
 }
 
-// out = x / sum of its row, or NaN where x == 0 and the sum is 0; one cell
-// per thread over rows x S
+// out = x / sum of its row, or NaN where x == 0 and the sum is 0: workgroup
+// (x, y) takes SCALE_DIV_ITEMS x 256 stocks of rows y, y + gridDim.y, ...
+constexpr int SCALE_DIV_ITEMS = 4;
 template <typename T, int LIN, int LOUT>
 __global__ __launch_bounds__(256) void scale_div(kun_cs::View in, kun_cs::View out, long S, long rows, int blk,
-                                                 const T *sums) {
-    const long i = (long)blockIdx.x * 256 + threadIdx.x;
-    if (i >= rows * S) return;
-    const long r = i / S, s = i - r * S;
-    const T x = ((const T *)in.ptr)[lay_at<LIN>(in, r + in.row0, s, S, blk)];
-    const T tot = sums[r];
-    ((T *)out.ptr)[lay_at<LOUT>(out, r + out.row0, s, S, blk)] = (x == T(0) && tot == T(0)) ? kun::qnan<T>() : x / tot;
+                                                 T *sums) {
+    const long s0 = (long)blockIdx.x * (256 * SCALE_DIV_ITEMS) + threadIdx.x;
+    const T *src = (const T *)in.ptr;
+    T *dst = (T *)out.ptr;
+    auto rows_of = [&](auto full) {
+        for (long r = blockIdx.y; r < rows; r += gridDim.y) {
+            const T tot = sums[r];
+            const double rc = sizeof(T) == 4 ? scale_rcs(sums, rows)[r] : 0.0;
+            T x[SCALE_DIV_ITEMS];
+#pragma unroll
+            for (int q = 0; q < SCALE_DIV_ITEMS; q++) {
+                const long s = s0 + q * 256;
+                x[q] = (full || s < S) ? src[lay_at<LIN>(in, r + in.row0, s, S, blk)] : T(0);
+            }
+#pragma unroll
+            for (int q = 0; q < SCALE_DIV_ITEMS; q++) {
+                const long s = s0 + q * 256;
+                if (full || s < S) dst[lay_at<LOUT>(out, r + out.row0, s, S, blk)] = scale_q(x[q], tot, rc);
+            }
+        }
+    };
+    // whole workgroups of stocks take the unguarded copy
+    if ((long)(blockIdx.x + 1) * (256 * SCALE_DIV_ITEMS) <= S)
+        rows_of(std::true_type{});
+    else
+        rows_of(std::false_type{});
 }
 
 template <typename T> __global__ __launch_bounds__(SCALE_BLOCK) void scale_segs(const kun_cs::Seg *segs) {
@@ -778,8 +820,10 @@ hipError_t launch_scale_split(const kun_cs::View &in, const kun_cs::View &out, l
     hipError_t e = sink_launch(sink, scale_sums<T, LIN>, dim3((unsigned)((rows + SCALE_SUM_ROWS - 1) / SCALE_SUM_ROWS)),
                                dim3(SCALE_BLOCK), in, S, rows, blk, sums);
     if (e != hipSuccess) return e;
-    return sink_launch(sink, scale_div<T, LIN, LOUT>, dim3((unsigned)((rows * S + 255) / 256)), dim3(256), in, out, S,
-                       rows, blk, (const T *)sums);
+    const long per = 256L * SCALE_DIV_ITEMS;
+    return sink_launch(sink, scale_div<T, LIN, LOUT>,
+                       dim3((unsigned)((S + per - 1) / per), (unsigned)(rows < 65535 ? rows : 65535)), dim3(256), in,
+                       out, S, rows, blk, sums);
 }
 
 template <typename T>
@@ -867,6 +911,11 @@ hipError_t launch_rank(int dtype, const View &in, const View &out, long S, long 
 hipError_t launch_rank(int dtype, const View &in, const View &out, long S, long rows, int blk, hipStream_t st) {
     StreamSink sink(st);
     return launch_rank(dtype, in, out, S, rows, blk, sink);
+}
+
+size_t scale_scratch_bytes(int dtype, long rows) {
+    const size_t n = (size_t)(rows > 1 ? rows : 1);
+    return dtype ? n * 8 : (n * 4 + 7) / 8 * 8 + n * 8;
 }
 
 hipError_t launch_scale(int dtype, const View &in, const View &out, long S, long rows, int blk, Sink &sink,

@@ -49,7 +49,9 @@ hipError_t launch_corr(const float *x, int xsts, long xT, long xrow0, const floa
                        long S, long start, long L, int blk, hipStream_t st);
 hipError_t launch_aggregate(int dtype, const void *in, const void *labels, void *const *outs, long S, long start,
                             long L, hipStream_t st);
-// scratch (rows values, may be null): row sums of the split form
+// scratch (scale_scratch_bytes, may be null): row sums of the split form
+// (and, float, their reciprocals in double)
+size_t scale_scratch_bytes(int dtype, long rows);
 hipError_t launch_scale(int dtype, const View &in, const View &out, long S, long rows, int blk, Sink &sink,
                         void *scratch = nullptr);
 hipError_t launch_scale(int dtype, const View &in, const View &out, long S, long rows, int blk, hipStream_t st,

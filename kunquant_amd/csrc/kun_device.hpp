@@ -57,14 +57,26 @@ template <typename T> __device__ __forceinline__ T vabs(T a) {
     using U = typename Bits<T>::U;
     return from_bits<T>(to_bits(a) & ~(U(1) << (Bits<T>::NBITS - 1)));
 }
-// x / c for a finite, nonzero float constant c, rounded exactly as IEEE
-// float division: rc = RN(1/c) in double, and the double product x * rc is
-// within 2^-28 ulp of x / c, while x / c is never nearer than ulp / 2^25 to
-// a midpoint between floats (the odd part of c's significand has at most
-// 24 bits) -- so rounding the product to float rounds x / c. Three VALU
-// operations instead of the division sequence; checked for every float x
-// and a set of constants by tests/native/div_const_check.c.
-__device__ __forceinline__ float div_const(float x, double rc) { return (float)((double)x * rc); }
+// x / c for a float c, rounded exactly as IEEE float division, given
+// rc = RN(1/c) in double. The double product x * rc is within 2^-52 of x / c
+// (relative). A normal float result is then exact: x / c is never a
+// midpoint between normal floats (a midpoint's odd part has 25 bits, x's at
+// most 24) and lies at least 2^-49 (relative) from one, so rounding the
+// product to float rounds x / c. A subnormal result is not: midpoints there
+// have short odd parts, x / c can be exactly one (c = 16350, x = 2047 * 8175
+// * 2^-149), and the product's error then picks a side instead of the even
+// neighbour. Those results (and only those: a zero product is never the
+// wrong side of a tie) take the division. Checked for every float x and a
+// set of divisors, tie-prone ones included, by
+// tests/native/div_const_check.c.
+__device__ __forceinline__ float div_rc(float x, double rc, float c) {
+    const float r = (float)((double)x * rc);
+    if (__builtin_expect(__builtin_isfpclass(r, 0x0090), 0)) return x / c;  // subnormal
+    return r;
+}
+// the bare product, for divisors whose midpoint quotients all round right
+// (every float x checked when the kernel is generated)
+__device__ __forceinline__ float div_const(float x, double rc, float) { return (float)((double)x * rc); }
 __device__ __forceinline__ float vsqrt(float a) { return __builtin_sqrtf(a); }
 __device__ __forceinline__ double vsqrt(double a) { return __builtin_sqrt(a); }
 __device__ __forceinline__ float fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
@@ -414,6 +426,21 @@ template <typename T, int W, bool IDX> struct SortedWinR {
 // `v` loaded from row `row` (clamped at 0 by the caller) is NaN when the row
 // lies before the run start -- no history before start (Ops.hpp:87-92)
 template <typename T> __device__ __forceinline__ T nan_before(T v, long row) { return row >= 0 ? v : qnan<T>(); }
+
+// Window read of a [rows][S] buffer at a row known to lie inside it: a raw
+// buffer load whose byte offset is the wave-uniform `roff` (row * S *
+// sizeof(T), kept in a scalar register) plus the lane's `so` (stock *
+// sizeof(T)), so the address costs no vector instruction. The caller
+// guarantees roff + so < 2^31. Word 3 of the descriptor is the gfx9
+// raw-buffer format; num_records is the maximum, so no load is clipped.
+template <typename T> __device__ __forceinline__ T wld(const T *base, unsigned so, long roff) {
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, -1, 0x00020000);
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)so, (int)roff, 0));
+    } else {
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)so, (int)roff, 0));
+    }
+}
 
 // element (row, stock) of a buffer in TS [rows][S] or STs [S/b][rows][b]
 template <int LAYOUT> __device__ __forceinline__ size_t at(long row, long s, long S, long rows, int blk) {

@@ -406,7 +406,7 @@ BatchLayout batch_layout(const KunModuleDesc *m, size_t S, size_t L, const Chunk
         const KunStageDesc &sd = m->stages[i];
         if (sd.kind == KUN_STAGE_SCALE) {
             lay.cs_scratch[i] = off;
-            off += align256(std::max<size_t>(1, L) * elem_size(m));
+            off += align256(kun_cs::scale_scratch_bytes(m->dtype, (long)L));
         }
         if (sd.kind != KUN_STAGE_TS) continue;
         if (sd.nin + sd.nout + sd.nmat > 256) throw Error("stage with too many buffers");

@@ -1,8 +1,11 @@
 /* Division of a float by a constant as a double product: for every float x
  * (all 2^32 bit patterns) and each constant c given on the command line,
- * checks that (float)((double)x * (1.0 / (double)c)) has the bits of the
- * IEEE float quotient x / c (NaNs: both NaN). Prints one line per constant
- * and exits 1 at the first mismatch. Threads split the bit patterns. */
+ * checks that r = (float)((double)x * (1.0 / (double)c)), replaced by x / c
+ * where r is subnormal (kun::div_rc), has the bits of the IEEE float
+ * quotient x / c (NaNs: both NaN). With -bare first, r is checked without
+ * the replacement. Prints one line per constant (mismatches of the checked
+ * form) and exits 1 if any constant has one. Threads split the bit
+ * patterns. */
 #include <math.h>
 #include <pthread.h>
 #include <stdint.h>
@@ -13,6 +16,7 @@
 #define NT 8
 
 typedef struct {
+    int bare;
     float c;
     double rc;
     uint64_t lo, hi;
@@ -29,7 +33,8 @@ static void *run(void *p) {
         uint32_t b = (uint32_t)u;
         memcpy(&x, &b, 4);
         const float q = x / c;
-        const float r = (float)((double)x * j->rc);
+        float r = (float)((double)x * j->rc);
+        if (!j->bare && fpclassify(r) == FP_SUBNORMAL) r = x / c;
         uint32_t bq, br;
         memcpy(&bq, &q, 4);
         memcpy(&br, &r, 4);
@@ -42,13 +47,15 @@ static void *run(void *p) {
 }
 
 int main(int argc, char **argv) {
-    int rc_all = 0;
-    for (int a = 1; a < argc; a++) {
+    int rc_all = 0, bare = 0, a0 = 1;
+    if (argc > 1 && strcmp(argv[1], "-bare") == 0) bare = 1, a0 = 2;
+    for (int a = a0; a < argc; a++) {
         const float c = strtof(argv[a], NULL);
         job_t jobs[NT];
         pthread_t th[NT];
         const uint64_t n = 1ull << 32, step = n / NT;
         for (int t = 0; t < NT; t++) {
+            jobs[t].bare = bare;
             jobs[t].c = c;
             jobs[t].rc = 1.0 / (double)c;
             jobs[t].lo = t * step;

@@ -161,6 +161,16 @@ def _cs_rows(S, dt, seed):
     rows.append(rng.standard_normal(S) * 10.0 ** rng.uniform(-30, 30, S))
     one = np.asarray(1.0, dt)
     rows.append(one + rng.integers(0, 5, S) * np.finfo(dt).eps)
+    # Scale's quotient exactly between two subnormal floats: the row sums to
+    # c = 2C (the subnormal cells vanish in the float sum) and x / c = o *
+    # 2^-150 for odd o, which rounds to the even neighbour
+    for C in (8175, 16307):
+        o = 2 * (((1 << 24) // C - 1) // 2) + 1 - 2 * np.arange(S - 1)
+        o = o[o > 0]
+        row = np.zeros(S)
+        row[0] = 2.0 * C
+        row[1:1 + len(o)] = (o * C).astype(np.float64) * 2.0 ** -149
+        rows.append(row)
     return np.ascontiguousarray(np.stack(rows).astype(dt))
 
 

@@ -38,8 +38,22 @@ from .wire import wire_table
 TS_BLOCK = 64
 MAX_STAGE_BUFS = 256
 LDS_LIMIT = 160 * 1024
-# LDS per 64-stock workgroup before history rings spill to HBM scratch
-LDS_BUDGET = int(os.environ.get("KUN_LDS_BUDGET", 40 * 1024))
+# LDS per 64-stock workgroup before history rings spill to HBM scratch, in
+# ring slots (one row of 64 stocks each): 80 slots are 20 KiB in fp32 and
+# 40 KiB in fp64, which measured fastest for both on the 100-factor graph
+# (more LDS per workgroup costs more in occupancy than the HBM re-reads it
+# saves). KUN_LDS_BUDGET gives bytes instead.
+LDS_SLOTS = 80
+LDS_BUDGET = int(os.environ["KUN_LDS_BUDGET"]) if os.environ.get("KUN_LDS_BUDGET") else None
+
+
+def lds_budget_for(module: ModuleIR, override: Optional[int] = None) -> int:
+    """LDS bytes per workgroup for `module`'s history rings."""
+    if override is not None:
+        return override
+    if LDS_BUDGET is not None:
+        return LDS_BUDGET
+    return LDS_SLOTS * TS_BLOCK * (8 if module.dtype == "double" else 4)
 # registers of prefetched loads per component, and the deepest prefetch
 # (rows) of a small sequential component
 PREFETCH_REGS = int(os.environ.get("KUN_PREFETCH_REGS", 64))
@@ -729,7 +743,7 @@ class CompEmitter(_Lowering):
     (BackRef, FastWindowedSum, regression sums), or a direct HBM load."""
 
     def __init__(self, module: ModuleIR, stage, plan: StagePlan, comp: Component, kind: int,
-                 lds_budget: int = LDS_BUDGET):
+                 lds_budget: Optional[int] = None):
         self._setup(module, stage)
         self.plan = plan
         self.kind = kind
@@ -774,7 +788,7 @@ class CompEmitter(_Lowering):
         self.ring: Dict[int, Tuple[int, int]] = {}
         self.tslot: Dict[int, int] = {}
         self.islot: Dict[int, int] = {}
-        cap = lds_budget // (self.esz * TS_BLOCK)
+        cap = lds_budget_for(module, lds_budget) // (self.esz * TS_BLOCK)
         n_lds = 0
         spilled = []
         for op in sorted(ring_need, key=lambda o: self.idx[o]):
@@ -1211,7 +1225,7 @@ class LevelKernel:
                  comps=None):
         self.kname = kname
         self.kind = level.kind
-        budget = LDS_BUDGET if lds_budget is None else lds_budget
+        budget = lds_budget_for(module, lds_budget)
         comps = level.comps if comps is None else comps
         if len(comps) > MAX_COMPS:
             raise RuntimeError(f"stage {stage.name}: {len(comps)} components in one kernel (limit {MAX_COMPS})")
@@ -1403,7 +1417,7 @@ def emit_module_parts(module: ModuleIR, nparts: int, lds_budget: int = None) -> 
         else:
             plan = stage_plan(module, st)
             kerns = []
-            budget = LDS_BUDGET if lds_budget is None else lds_budget
+            budget = lds_budget_for(module, lds_budget)
             for level in plan.levels:
                 # one launch per LDS class, so that components with small
                 # rings are not held to the occupancy of the largest

@@ -58,14 +58,18 @@ def lds_budget_for(module: ModuleIR, override: Optional[int] = None) -> int:
 # (rows) of a small sequential component
 PREFETCH_REGS = int(os.environ.get("KUN_PREFETCH_REGS", 64))
 SEQ_PREFETCH_MAX = int(os.environ.get("KUN_SEQ_PREFETCH_MAX", 16))
-# unroll factor of window loops
-LOOP_UNROLL = int(os.environ.get("KUN_LOOP_UNROLL", 8))
+# unroll factor of window loops (10: the common 10/20/30/60-row windows
+# split into whole groups; measured best of 6/8/10/12 on the 100-factor
+# graph)
+LOOP_UNROLL = int(os.environ.get("KUN_LOOP_UNROLL", 10))
 SORTED_REGS = 256  # register budget (values) for the sorted windows of one sequential component
 # row-chunked components: rows unrolled per group by estimated cost per row
 # (first threshold met wins; lighter components unroll 8 rows)
 PAR_UNROLL_BY_COST = ((1024, 1), (256, 2), (64, 4))
 # row-chunked components with window loops: rows per interleaved group
-INTERLEAVE_MIN, INTERLEAVE_MAX = 2, 4
+# (2: at two waves per SIMD, wider groups cost more in registers than they
+# gain in overlapped chains; measured against 1, 3 and 4)
+INTERLEAVE_MIN, INTERLEAVE_MAX = 2, 2
 
 _BIN = {"Add": "+", "Sub": "-", "Mul": "*", "Div": "/"}
 _CMP = {"GreaterThan": ">", "GreaterEqual": ">=", "LessThan": "<", "LessEqual": "<=", "Equals": "=="}
@@ -190,16 +194,22 @@ def loop_code(w: int, body: List[str], unroll: int) -> List[str]:
             out.append(x)
         return out
 
+    def group(n: int) -> List[str]:
+        copies = [copy(u) for u in range(n)]
+        G = []
+        for c in copies:
+            G.extend(c[i] for i in hoist)
+        for c in copies:
+            G.extend(c[i] for i in range(len(c)) if i not in hoist)
+        return G
+
     L = ["{", "int it0 = " + str(w - 1) + ";", f"for (; it0 >= {U - 1}; it0 -= {U}) {{"]
-    copies = [copy(u) for u in range(U)]
-    for c in copies:
-        L.extend(c[i] for i in hoist)
-    for c in copies:
-        L.extend(c[i] for i in range(len(c)) if i not in hoist)
+    L.extend(group(U))
     L.append("}")
     if w % U:
-        L.append("for (int iter = it0; iter >= 0; --iter) {")
-        L.extend(body)
+        # the last w % U iterations (it0 == w % U - 1 here), unrolled alike
+        L.append("{")
+        L.extend(group(w % U))
         L.append("}")
     L.append("}")
     return L
@@ -951,7 +961,11 @@ class CompEmitter(_Lowering):
     def lag_load(self, op, row: str) -> str:
         """Row `row` of a row source, NaN before the run start; the load is
         unconditional (row clamped at 0) so the row loop has no branch."""
-        return f"kun::nan_before({self.src_load(op, f'(({row}) < 0 ? 0 : ({row}))')}, {row})"
+        return f"kun::nan_before({self.lag_raw(op, row)}, {row})"
+
+    def lag_raw(self, op, row: str) -> str:
+        """The load of lag_load without its NaN select (row clamped at 0)."""
+        return self.src_load(op, f"(({row}) < 0 ? 0 : ({row}))")
 
     # set while lowering the copy of a window loop that runs when every row
     # the loop reads lies inside the run (see loop_parts)
@@ -1060,11 +1074,19 @@ class CompEmitter(_Lowering):
         nxt = f"(i + {D} < c1 ? i + {D} : c1 - 1)"
         L.extend(self.ring_positions())
 
-        def take(name: str, load: str) -> str:
+        # slots are refilled at the end of the row, once the row has used
+        # their values: a refill issued while the old value is still live
+        # needs another register, and the loop then copies the slots on
+        # every iteration, waiting for their loads
+        refill: List[str] = []
+
+        def take(name: str, load: str, wrap=lambda x: x) -> str:
             if P > 1:
-                return f"{name}{q}; {name}{q} = {load};"
+                refill.append(f"{name}{q} = {load};")
+                return f"{wrap(f'{name}{q}')};"
             moves = "".join(f" {name}{j} = {name}{j + 1};" for j in range(D - 1))
-            return f"{name}0;{moves} {name}{D - 1} = {load};"
+            refill.append(f"{moves} {name}{D - 1} = {load};".strip())
+            return f"{wrap(f'{name}0')};"
 
         for op in self.srcs:
             k = self.idx[op]
@@ -1075,8 +1097,12 @@ class CompEmitter(_Lowering):
                 L.extend(self.ring_push(k))
         for (k, off) in self.lags:
             op = self.st.func.ops[k]
-            L.append(f"const {T} lg{k}_{off} = " + take(f"pl{k}_{off}_", self.lag_load(op, f"{nxt} - {off}")))
-        return [("lines", L)] + self.step_segments(skip=set(self.srcs) | self.single)
+            # the slot holds the raw load; the NaN select for rows before
+            # the run start is applied here, where the value is used, so
+            # the load stays in flight until then
+            L.append(f"const {T} lg{k}_{off} = " + take(f"pl{k}_{off}_", self.lag_raw(op, f"{nxt} - {off}"),
+                                                          lambda x: f"kun::nan_before({x}, i - {off})"))
+        return [("lines", L)] + self.step_segments(skip=set(self.srcs) | self.single) + [("lines", refill)]
 
     def group_lines(self, P: int) -> List[str]:
         """Rows i0 .. i0 + P - 1 interleaved op by op: each op is emitted for
@@ -1167,7 +1193,7 @@ class CompEmitter(_Lowering):
                 if self._src_row_used(op):
                     L.append(f"{T} pf{self.idx[op]}_{q} = {self.src_load(op, first)};")
             for (k, off) in self.lags:
-                L.append(f"{T} pl{k}_{off}_{q} = {self.lag_load(self.st.func.ops[k], f'{first} - {off}')};")
+                L.append(f"{T} pl{k}_{off}_{q} = {self.lag_raw(self.st.func.ops[k], f'{first} - {off}')};")
         if self.kind == PAR:
             # halo rows rebuild history and store nothing; stored rows follow
             # in a phase of their own so neither phase branches on the row

@@ -236,6 +236,7 @@ def _flatten(segs, unroll: int) -> List[str]:
 
 
 _RENAME = [(re.compile(r"\br(\d+)_"), r"r\1q{q}_"), (re.compile(r"\blg(\d+)_(\d+)\b"), r"lg\1_\2q{q}"),
+           (re.compile(r"\blr(\d+)_(\d+)\b"), r"lr\1_\2q{q}"),
            (re.compile(r"\brp(\d+)\b"), r"rp\1q{q}"), (re.compile(r"\bi\b"), "iq{q}")]
 _VNAME = re.compile(r"\bv(\d+)\b")
 
@@ -984,9 +985,18 @@ class CompEmitter(_Lowering):
             return f"kun::wld(b{b.idx}, so, ({r}) * A.S * {e})"
         return f"kun::wld(m{k}, so, ({row}) * A.S * {e})"
 
+    # ring reads of earlier rows hoisted to the top of the row being
+    # emitted (see row_segments): None when not collecting
+    _hoist: Optional[Dict[str, str]] = None
+    _hoist_min = 1
+
     def win(self, src, off) -> str:
         k = self.idx[src]
         if src in self.computed:
+            if self._hoist is not None and isinstance(off, int) and off >= self._hoist_min:
+                name = f"lr{k}_{off}"
+                self._hoist.setdefault(name, f"const {self.T} {name} = {self.ring_read(k, off)};")
+                return name
             return self.ring_read(k, off)
         if off != "iter":
             # single-offset read of a row source: its prefetched lag load (a
@@ -1066,8 +1076,12 @@ class CompEmitter(_Lowering):
         P == 1 the slots form a shift register `depth` rows deep."""
         return _flatten(self.row_segments(q, P), self.loop_unroll)
 
-    def row_segments(self, q: int, P: int) -> list:
-        """row_lines as segments (see step_segments)."""
+    def row_segments(self, q: int, P: int, grouped: bool = False) -> list:
+        """row_lines as segments (see step_segments). Single-offset ring
+        reads of rows the ring already holds when the row starts (offset >=
+        1; >= P when the P rows of a group interleave) are issued at the top
+        of the row, before the row's own ring stores, so their LDS latency
+        overlaps the row's work instead of its dependent chain."""
         T = self.T
         L = []
         D = P if P > 1 else self.depth
@@ -1102,7 +1116,13 @@ class CompEmitter(_Lowering):
             # the load stays in flight until then
             L.append(f"const {T} lg{k}_{off} = " + take(f"pl{k}_{off}_", self.lag_raw(op, f"{nxt} - {off}"),
                                                           lambda x: f"kun::nan_before({x}, i - {off})"))
-        return [("lines", L)] + self.step_segments(skip=set(self.srcs) | self.single) + [("lines", refill)]
+        self._hoist, self._hoist_min = {}, (P if grouped else 1)
+        try:
+            segs = self.step_segments(skip=set(self.srcs) | self.single)
+            L.extend(self._hoist.values())
+        finally:
+            self._hoist = None
+        return [("lines", L)] + segs + [("lines", refill)]
 
     def group_lines(self, P: int) -> List[str]:
         """Rows i0 .. i0 + P - 1 interleaved op by op: each op is emitted for
@@ -1112,7 +1132,7 @@ class CompEmitter(_Lowering):
         rings hold P - 1 extra slots because the later rows' pushes precede
         the earlier rows' window reads."""
         keep = {self.idx[op] for op in self.single}
-        segs = [self.row_segments(q, P) for q in range(P)]
+        segs = [self.row_segments(q, P, grouped=True) for q in range(P)]
         L = [f"const long iq{q} = i0 + {q};" for q in range(P)]
         for j, sg in enumerate(segs[0]):
             if sg[0] == "lines":

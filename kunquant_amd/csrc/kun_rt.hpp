@@ -101,6 +101,10 @@ struct ChunkPolicy {
     long target_workgroups = 32768;
     long halo_factor = 3;
     long min_rows = 64;
+    // a kernel left with fewer workgroups than this (16 per CU) splits its
+    // components' rows further, down to chunks of one halo (at least 16
+    // rows): the large-halo kernels otherwise run a few hundred long waves
+    long min_workgroups = 4096;
     static ChunkPolicy from_env();
 };
 BatchLayout batch_layout(const KunModuleDesc *m, size_t S, size_t L, const ChunkPolicy &pol, int lanes);

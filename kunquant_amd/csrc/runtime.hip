@@ -253,6 +253,7 @@ ChunkPolicy ChunkPolicy::from_env() {
     p.target_workgroups = env_long("KUN_TARGET_WORKGROUPS", p.target_workgroups);
     p.halo_factor = env_long("KUN_HALO_FACTOR", p.halo_factor);
     p.min_rows = env_long("KUN_MIN_CHUNK_ROWS", p.min_rows);
+    p.min_workgroups = env_long("KUN_MIN_KERNEL_WORKGROUPS", p.min_workgroups);
     return p;
 }
 
@@ -281,6 +282,21 @@ static void plan_kernel(const KunKernelDesc &kd, size_t S, size_t L, const Chunk
             nch[z] = ((long)L + lc - 1) / lc;
             rows[z] = ((long)L + nch[z] - 1) / nch[z];
             nch[z] = ((long)L + rows[z] - 1) / rows[z];
+        }
+        long wgs = 0;
+        for (int z = 0; z < nc; z++) wgs += nsb * nch[z];
+        if (wgs < pol.min_workgroups) {
+            const long want = (pol.min_workgroups + nsb * nc - 1) / (nsb * nc);
+            for (int z = 0; z < nc; z++) {
+                if (nch[z] >= want) continue;
+                const long floor_rows = std::max(16L, (long)kd.comp_halo[z]);
+                long lc = std::max(floor_rows, ((long)L + want - 1) / want);
+                lc = std::min((long)L, lc);
+                const long n = ((long)L + lc - 1) / lc;
+                if (n <= nch[z]) continue;
+                rows[z] = ((long)L + n - 1) / n;
+                nch[z] = ((long)L + rows[z] - 1) / rows[z];
+            }
         }
     }
     std::vector<int> order(nc);

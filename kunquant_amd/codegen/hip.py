@@ -66,6 +66,12 @@ SORTED_REGS = 256  # register budget (values) for the sorted windows of one sequ
 # row-chunked components: rows unrolled per group by estimated cost per row
 # (first threshold met wins; lighter components unroll 8 rows)
 PAR_UNROLL_BY_COST = ((1024, 1), (256, 2), (64, 4))
+# scheduling barrier between the unrolled rows of a non-interleaved loop
+# (mask of the instruction kinds allowed across it), or None. 0x9C lets
+# scalar, memory and LDS instructions cross but no vector ALU: without it
+# the scheduler lifts every row's NaN select of its lag value to the top of
+# the 16-row iteration, where it waits for loads issued a row earlier
+ROW_BARRIER = 0x9C
 # row-chunked components with window loops: rows per interleaved group
 # (2: at two waves per SIMD, wider groups cost more in registers than they
 # gain in overlapped chains; measured against 1, 3 and 4)
@@ -1251,6 +1257,8 @@ class CompEmitter(_Lowering):
             for q in range(P):
                 L.append(f"    {{ const long i = i0 + {q};")
                 L.extend("      " + x for x in self.row_lines(q, P))
+                if ROW_BARRIER is not None:
+                    L.append(f"      __builtin_amdgcn_sched_barrier({ROW_BARRIER});")
                 L.append("    }")
         L.append("  }")
         for q in range(P - 1):

@@ -51,3 +51,4 @@ static inline uint64_t __builtin_amdgcn_raw_buffer_load_b64(__amdgpu_buffer_rsrc
     memcpy(&v, r.base + (uint32_t)vo + (uint32_t)so, 8);
     return v;
 }
+static inline void __builtin_amdgcn_sched_barrier(int) {}

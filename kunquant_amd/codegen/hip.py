@@ -58,6 +58,11 @@ def lds_budget_for(module: ModuleIR, override: Optional[int] = None) -> int:
 # (rows) of a small sequential component
 PREFETCH_REGS = int(os.environ.get("KUN_PREFETCH_REGS", 64))
 SEQ_PREFETCH_MAX = int(os.environ.get("KUN_SEQ_PREFETCH_MAX", 16))
+# rows unrolled (and prefetched) in sequential components of more than 60
+# ops without window loops (1 before: the fp64 100-factor graph's
+# accumulator webs then waited for every row's loads; 2 / 4 rows measured
+# 77.7 / 77.8 against 78.9 ms, Alpha158 fp64 17.9 / 17.7 against 18.3 ms)
+SEQ_HEAVY_UNROLL = 4
 # unroll factor of window loops (10: the common 10/20/30/60-row windows
 # split into whole groups; measured best of 6/8/10/12 on the 100-factor
 # graph)
@@ -864,8 +869,10 @@ class CompEmitter(_Lowering):
         nload = max(1, sum(1 for op in self.srcs if self._src_row_used(op)) + len(self.lags))
         if kind == PAR:
             p = next((pp for c, pp in PAR_UNROLL_BY_COST if self.cost >= c), 8)
-        elif has_loop or nops > 60:
+        elif has_loop:
             p = 1
+        elif nops > 60:
+            p = SEQ_HEAVY_UNROLL
         else:
             p = SEQ_PREFETCH_MAX if nops <= 24 else 4
         self.prefetch = max(1, min(p, PREFETCH_REGS // nload))
@@ -1106,9 +1113,11 @@ class CompEmitter(_Lowering):
             if P > 1:
                 refill.append(f"{name}{q} = {load};")
                 return f"{wrap(f'{name}{q}')};"
+            # one slot (or a shift register): the refill follows the take, so
+            # the load overlaps this row's work (at the row's end it would
+            # be waited for at the top of the next row)
             moves = "".join(f" {name}{j} = {name}{j + 1};" for j in range(D - 1))
-            refill.append(f"{moves} {name}{D - 1} = {load};".strip())
-            return f"{wrap(f'{name}0')};"
+            return f"{wrap(f'{name}0')};{moves} {name}{D - 1} = {load};"
 
         for op in self.srcs:
             k = self.idx[op]

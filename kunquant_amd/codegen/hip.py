@@ -63,6 +63,9 @@ SEQ_PREFETCH_MAX = int(os.environ.get("KUN_SEQ_PREFETCH_MAX", 16))
 # accumulator webs then waited for every row's loads; 2 / 4 rows measured
 # 77.7 / 77.8 against 78.9 ms, Alpha158 fp64 17.9 / 17.7 against 18.3 ms)
 SEQ_HEAVY_UNROLL = 4
+# rows unrolled in sequential components with window loops (2 / 4 rows
+# measured within 2% of 1 on the fp64 graphs, either way)
+SEQ_LOOP_UNROLL = 1
 # unroll factor of window loops (10: the common 10/20/30/60-row windows
 # split into whole groups; measured best of 6/8/10/12 on the 100-factor
 # graph)
@@ -870,7 +873,7 @@ class CompEmitter(_Lowering):
         if kind == PAR:
             p = next((pp for c, pp in PAR_UNROLL_BY_COST if self.cost >= c), 8)
         elif has_loop:
-            p = 1
+            p = SEQ_LOOP_UNROLL
         elif nops > 60:
             p = SEQ_HEAVY_UNROLL
         else:

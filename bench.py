@@ -369,23 +369,38 @@ def dag_timeline(kr, mod, ptrs, S, T, wall_ms):
     ser.enable_profiling(True)
     kr.runGraphDevice(ser, mod, ptrs, S, T, 0, T)
     ser.synchronize()
-    alone = {}
-    for i, _ in enumerate(mod.stages()):
-        for j in range(max(1, len(mod.kernels(i)))):
+    alone, kind_of = {}, {}
+    for i, (_, kind, _, _) in enumerate(mod.stages()):
+        kinfo = mod.kernels(i)
+        for j in range(max(1, len(kinfo))):
             tot, n = ser.kernel_time(mod, i, j)
             alone[(i, j)] = tot / max(n, 1)
+            kind_of[(i, j)] = (("chunked" if kinfo[j][0] == 0 else "sequential") + " time-series" if kind == 0
+                               else {1: "rank", 2: "scale", 3: "generic cross-sectional"}.get(kind, "other"))
     ser.enable_profiling(False)
-    finish = {}
+    finish, key_of, prev = {}, {}, {}
     for line in kr.describeGraphBatch([(mod, S, T, 0, T)]).splitlines():
         if not line.startswith("dag "):
             continue
         head, deps = line.split(":")
         _, _, node, stage, kern, _ = head.split()
-        start = max((finish[int(d)] for d in deps.split()), default=0.0)
-        finish[int(node)] = start + alone.get((int(stage), int(kern)), 0.0)
+        node = int(node)
+        preds = [int(d) for d in deps.split()]
+        best = max(preds, key=lambda d: finish[d], default=None)
+        key_of[node] = (int(stage), int(kern))
+        prev[node] = best
+        finish[node] = (finish[best] if best is not None else 0.0) + alone.get(key_of[node], 0.0)
     serial = sum(alone.values())
-    crit = max(finish.values())
+    last = max(finish, key=lambda n: finish[n])
+    crit = finish[last]
+    by_kind: dict = {}
+    node = last
+    while node is not None:
+        k = kind_of.get(key_of[node], "other")
+        by_kind[k] = by_kind.get(k, 0.0) + alone.get(key_of[node], 0.0)
+        node = prev[node]
     return {"serial_ms": serial, "critical_path_ms": crit, "dag_nodes": len(finish),
+            "critical_path_ms_by_kind": {k: round(v, 3) for k, v in by_kind.items()},
             "wall_ms": wall_ms, "work_over_wall": serial / wall_ms,
             "note": "kernels timed alone; critical path = longest launch-DAG path of those times"}
 

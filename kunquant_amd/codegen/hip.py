@@ -47,12 +47,22 @@ LDS_SLOTS = 80
 LDS_BUDGET = int(os.environ["KUN_LDS_BUDGET"]) if os.environ.get("KUN_LDS_BUDGET") else None
 
 
-def lds_budget_for(module: ModuleIR, override: Optional[int] = None) -> int:
-    """LDS bytes per workgroup for `module`'s history rings."""
+# Sequential components run one workgroup per 64 stocks, far fewer than the
+# GPU holds, so occupancy costs them nothing: their rings get up to 80 KiB
+# (two workgroups per CU), and a window loop over a computed value reads
+# LDS instead of HBM scratch on every iteration of every row
+SEQ_LDS_BUDGET = 80 * 1024
+
+
+def lds_budget_for(module: ModuleIR, override: Optional[int] = None, seq: bool = False) -> int:
+    """LDS bytes per workgroup for `module`'s history rings (`seq`: of a
+    sequential component)."""
     if override is not None:
         return override
     if LDS_BUDGET is not None:
         return LDS_BUDGET
+    if seq:
+        return SEQ_LDS_BUDGET
     return LDS_SLOTS * TS_BLOCK * (8 if module.dtype == "double" else 4)
 # registers of prefetched loads per component, and the deepest prefetch
 # (rows) of a small sequential component
@@ -99,8 +109,9 @@ _OP_COST = {"Div": 10, "DivConst": 10, "Sqrt": 8, "Log": 24, "Exp": 24, "IterVal
             "WindowedLinearRegression": 30, "SkipListState": 60, "ForeachBackWindow": 2}
 MAX_COMPS = 64
 # upper LDS bytes of the launch classes a level's components are split into
-# (16, 8, 4 and 1 workgroups of 64 lanes per CU)
-LDS_CLASSES = (0, 10 * 1024, 20 * 1024, 40 * 1024, LDS_LIMIT)
+# (16, 8, 4, 2 and 1 workgroups of 64 lanes per CU; the 80 KiB class is
+# where sequential components with large rings go)
+LDS_CLASSES = (0, 10 * 1024, 20 * 1024, 40 * 1024, 80 * 1024, LDS_LIMIT)
 
 
 def c_literal(v, T: str) -> str:
@@ -815,7 +826,7 @@ class CompEmitter(_Lowering):
         self.ring: Dict[int, Tuple[int, int]] = {}
         self.tslot: Dict[int, int] = {}
         self.islot: Dict[int, int] = {}
-        cap = lds_budget_for(module, lds_budget) // (self.esz * TS_BLOCK)
+        cap = lds_budget_for(module, lds_budget, kind == SEQ) // (self.esz * TS_BLOCK)
         n_lds = 0
         spilled = []
         for op in sorted(ring_need, key=lambda o: self.idx[o]):
@@ -1293,11 +1304,10 @@ class LevelKernel:
                  comps=None):
         self.kname = kname
         self.kind = level.kind
-        budget = lds_budget_for(module, lds_budget)
         comps = level.comps if comps is None else comps
         if len(comps) > MAX_COMPS:
             raise RuntimeError(f"stage {stage.name}: {len(comps)} components in one kernel (limit {MAX_COMPS})")
-        self.comps = [CompEmitter(module, stage, plan, c, level.kind, budget) for c in comps]
+        self.comps = [CompEmitter(module, stage, plan, c, level.kind, lds_budget) for c in comps]
         self.T = self.comps[0].T
         self.halo = max(c.halo for c in self.comps)
         self.lds_bytes = max(c.lds_bytes for c in self.comps)
@@ -1485,16 +1495,19 @@ def emit_module_parts(module: ModuleIR, nparts: int, lds_budget: int = None) -> 
         else:
             plan = stage_plan(module, st)
             kerns = []
-            budget = lds_budget_for(module, lds_budget)
             for level in plan.levels:
                 # one launch per LDS class, so that components with small
                 # rings are not held to the occupancy of the largest
                 classes: Dict[int, list] = {}
                 for c in level.comps:
-                    cls = 0
-                    if level.kind == PAR:  # sequential levels are latency-bound: one launch
-                        lds = CompEmitter(module, st, plan, c, level.kind, budget).lds_bytes
+                    lds = CompEmitter(module, st, plan, c, level.kind, lds_budget).lds_bytes
+                    if level.kind == PAR:
                         cls = next(j for j, lim in enumerate(LDS_CLASSES) if lds <= lim)
+                    else:
+                        # sequential levels stay one launch (they are
+                        # latency-bound) but for components past 40 KiB,
+                        # which would hold every workgroup of it to two per CU
+                        cls = 0 if lds <= 40 * 1024 else 1
                     classes.setdefault(cls, []).append(c)
                 groups = []
                 for cls in sorted(classes):
